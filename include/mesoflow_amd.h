@@ -1,0 +1,243 @@
+/*
+ * mesoflow_amd.h -- C-ABI of the MI355X-native RHS + RK3 path of mesoflow.
+ *
+ * Every entry point has C linkage, takes plain pointers and sizes, and
+ * returns an int status (MF_OK == 0).  No exception and no abort crosses
+ * this boundary: where the reference calls amrex::Abort the device sets an
+ * error word that the host turns into one of the MF_ERR_* codes below.
+ *
+ * Layout of every state view (mf_array4) is the reference's amrex::Array4:
+ * component-major, x fastest, FP64,
+ *     p[(i-begin.x) + (j-begin.y)*jstride + (k-begin.z)*kstride + n*nstride]
+ * with the component order of Source/globalDefines.H:19-32
+ * (0-4 rho,rhou,rhov,rhow,rhoE; 5-10 density,velx,vely,velz,pressure,temperature;
+ *  11 volume fraction; 12.. species concentrations in mol/m^3).
+ *
+ * A context (mf_ctx) is not thread-safe: use one per host thread or stream.
+ */
+#ifndef MESOFLOW_AMD_H
+#define MESOFLOW_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MF_MAX_SPECIES   16
+#define MF_MAX_REACTIONS 8
+#define MF_NGROW         3      /* num_grow, Source/mflo_evolve.cpp:253 */
+
+/* component indices, Source/globalDefines.H:19-32 */
+enum {
+    MF_RHO = 0, MF_RHOU = 1, MF_RHOV = 2, MF_RHOW = 3, MF_RHOE = 4,
+    MF_DENS = 5, MF_VELX = 6, MF_VELY = 7, MF_VELZ = 8, MF_PRES = 9,
+    MF_TEMP = 10, MF_VFRAC = 11, MF_FLO_NVARS = 12
+};
+
+/* status codes (replace the amrex::Abort sites of the reference) */
+enum {
+    MF_OK = 0,
+    MF_ERR_NAN_DSDT = 1,        /* Source/Src_3d/div_3d.H:72-75            */
+    MF_ERR_NEG_PRESSURE = 2,    /* Source/mflo_evolve.cpp:162-167          */
+    MF_ERR_UNKNOWN_SCHEME = 3,  /* Source/Src_3d/compute_flux_3d.H:273-276 */
+    MF_ERR_BAD_BC = 4,          /* Source/mflo.cpp:113,141                 */
+    MF_ERR_BAD_ARG = 5,
+    MF_ERR_HIP = 6,
+    MF_ERR_UNSUPPORTED = 7,
+    MF_ERR_NO_DEVICE = 8,
+    MF_ERR_TOO_LARGE = 9
+};
+
+/* boundary codes of mflo.lo_bc / mflo.hi_bc (legend in every inputs file,
+ * e.g. tests/Cartesian/ShockTube_N2/inputs_x:30-36; mapping mflo.cpp:70-143) */
+enum { MF_BC_PERIODIC = 0, MF_BC_FOEXTRAP = 2, MF_BC_EXT_DIR = 3, MF_BC_WALL = 6 };
+
+/* compile-time plugin families built into the library (the reference selects
+ * its plugin headers per case through Make.package, e.g.
+ * tests/Cartesian/ShockTube_N2/Make.package:1-2) */
+enum {
+    MF_PLUGIN_SHOCKTUBE_N2 = 1,      /* tests/Cartesian/ShockTube_N2 headers, NUM_SPECIES=1 */
+    MF_PLUGIN_CATALYST_CYLINDER = 2, /* models/catalystCylinder headers, NUM_SPECIES=4  */
+    MF_PLUGIN_POROUS_ARRHENIUS = 3   /* mixture thermo of models/catalystCylinder/thermo.H
+                                        with Arrhenius mass-action kinetics, NS = 5 or 10 */
+};
+
+/* amrex::Array4<Real> (a view; the caller owns the memory). begin is the
+ * inclusive low corner (ghost cells included), end the exclusive high one. */
+typedef struct mf_array4 {
+    double *p;
+    int64_t jstride, kstride, nstride;
+    int32_t begin[3];
+    int32_t end[3];
+    int32_t ncomp;
+    int32_t pad_;
+} mf_array4;
+
+/* amrex::Box (cell-centred, inclusive bounds) */
+typedef struct mf_box {
+    int32_t lo[3];
+    int32_t hi[3];
+} mf_box;
+
+/* geometry.* keys: prob_lo, prob_hi, amr.n_cell, is_periodic */
+typedef struct mf_geom {
+    double prob_lo[3];
+    double prob_hi[3];
+    int32_t n_cell[3];
+    int32_t is_periodic[3];
+} mf_geom;
+
+/* mflo.* runtime parameters (Source/mflo.cpp:263-307, defaults mflo.H:226-259) */
+typedef struct mf_params {
+    int32_t order_hyp;          /* 1, 2 (default), 4, 5                         */
+    int32_t do_ns;              /* solve_navier_stokes                          */
+    int32_t species_in_solid;
+    int32_t conj_ht;            /* conjugate_heat_transfer                       */
+    int32_t using_bg_inertgas;
+    int32_t clip_species;
+    int32_t track_residual_norms;
+    int32_t do_chemistry;       /* 1: Advance_chemistry_implicit after the flow
+                                   RK3 (EvolveAMR default); 0: only_flow         */
+    double dissfactor;
+    double cfl;
+    double stop_time;           /* stop_time clamp of ComputeDt                  */
+    int32_t lo_bc[3];
+    int32_t hi_bc[3];
+    int32_t fixed_dt;           /* 1: use dt_fixed instead of the CFL estimate   */
+    int32_t pad_;
+    double dt_fixed;
+} mf_params;
+
+/* Runtime parameter tables of the compile-time plugins. In the reference
+ * these live in managed globals (e.g. tests/Cartesian/ShockTube_N2/species.H:18-19,
+ * userfuncs.cpp:3-14); here they are passed by value into the kernels. */
+typedef struct mf_plugin_params {
+    int32_t kind;                /* MF_PLUGIN_*                                 */
+    int32_t nspec;               /* NUM_SPECIES                                 */
+    int32_t ngas;                /* NUM_GAS_SPECIES: species 0..ngas-1 are gas  */
+    int32_t nreact;              /* NUM_REACTIONS                               */
+    double R_univ;               /* gas constant used by the thermo plugin      */
+    double molwt[MF_MAX_SPECIES];        /* kg/mol (species.cpp)               */
+    double gamma[MF_MAX_SPECIES];        /* gas species ratio of heats         */
+    double advect_flag[MF_MAX_SPECIES];  /* mflo_species::advect_flags         */
+    double visc, thcond;                 /* compute_fluid_transport            */
+    double dcoef_gas[MF_MAX_SPECIES];    /* compute_spec_dcoeff, vfrac == 1    */
+    double dcoef_solid[MF_MAX_SPECIES];  /* compute_spec_dcoeff, vfrac == 0    */
+    double stoich[MF_MAX_REACTIONS][MF_MAX_SPECIES]; /* rxnarray              */
+    double log_A[MF_MAX_REACTIONS];      /* ln of the pre-exponential          */
+    double Ea[MF_MAX_REACTIONS];         /* activation energy, J/mol           */
+    double kconst;                       /* constant rate of catalystCylinder  */
+    /* problem state read by externalbc / initdata */
+    double left_p, left_rho, left_T;     /* ShockTube p_l, rho_l, T_l          */
+    double right_p, right_rho, right_T;  /* ShockTube p_r, rho_r, T_r          */
+    double left_spec[MF_MAX_SPECIES];
+    double right_spec[MF_MAX_SPECIES];
+    double in_vel, in_temp, in_rho, in_pres;  /* inflow state (x-lo ext_dir)   */
+    double in_spec[MF_MAX_SPECIES];
+    double out_pres;                     /* catalystCylinder outflow p0        */
+    double siteconc;
+} mf_plugin_params;
+
+/* ------------------------------------------------------------------------ */
+/* Context                                                                    */
+/* ------------------------------------------------------------------------ */
+const char *mf_status_string(int status);
+int mf_device_count(int *count);
+
+/* Creates a context bound to the current HIP device. Checks the plugin/table
+ * combination (MF_ERR_UNSUPPORTED), the scheme (MF_ERR_UNKNOWN_SCHEME) and the
+ * boundary codes (MF_ERR_BAD_BC). */
+int mf_ctx_create(const mf_plugin_params *plugin, const mf_params *params,
+                  const mf_geom *geom, void **ctx_out);
+int mf_ctx_destroy(void *ctx);
+/* All later work of the context is enqueued on this stream (NULL = default). */
+int mf_ctx_set_stream(void *ctx, void *hip_stream);
+int mf_ctx_nvars(void *ctx, int *nv);          /* TOTAL_NVARS = 12 + NUM_SPECIES */
+/* Waits for the context's stream and returns (and clears) the device error word. */
+int mf_ctx_sync(void *ctx);
+
+/* ------------------------------------------------------------------------ */
+/* Plain per-box forms: the body of the MFIter loop of                        */
+/* mflo::compute_dsdt_flow (Source/mflo_evolve.cpp:579-652) and its neighbours */
+/* ------------------------------------------------------------------------ */
+
+/* update_cutcells over the valid box bx, in place on sborder
+ * (Source/mflo_evolve.cpp:509-517 -> Source/Src_3d/ib_utils.H:271-436). */
+int mf_box_cutcells(void *ctx, mf_box bx, mf_array4 sborder, double time);
+
+/* Per-box right-hand side: transport, species diffusion coefficients, fluid
+ * source, the three face-flux sweeps and update_residual
+ * (Source/mflo_evolve.cpp:602-651; Source/Src_3d/compute_flux_3d.H:86-430,
+ * Source/Src_3d/div_3d.H:13-76). sborder must hold bx grown by MF_NGROW
+ * (already ghost-filled and cut-cell updated). dsdt covers bx. flux may be
+ * NULL or point to three face-centred views (growHi(bx,d,1)). */
+int mf_box_rhs(void *ctx, mf_box bx, mf_array4 sborder, mf_array4 dsdt,
+               const mf_array4 *flux3, double time);
+
+/* The MultiFab arithmetic of one SSP-RK3 stage on the valid box
+ * (Source/mflo_evolve.cpp:283, 296-298, 309-311), same operation order:
+ *   stage 1: s_new = 1*src + dt*dsdt                    (src = Sborder)
+ *   stage 2: s_new = ((s_new + dt*dsdt) + 3*src) * 1/4  (src = S_old)
+ *   stage 3: s_new = (src + 2*(s_new + dt*dsdt)) * third (src = S_old) */
+int mf_box_rk3_combine(void *ctx, int stage, mf_box bx, mf_array4 s_new,
+                       mf_array4 src, mf_array4 dsdt, double dt);
+
+/* ------------------------------------------------------------------------ */
+/* Level forms: one AMR level of boxes stored as one domain-sized array with  */
+/* MF_NGROW ghost layers. Boxes must tile the domain (level 0).               */
+/* ------------------------------------------------------------------------ */
+int mf_level_create(void *ctx, const mf_box *boxes, int nboxes, void **level_out);
+int mf_level_destroy(void *level);
+/* State of the valid domain, [nv][nz][ny][nx] doubles, host or device memory. */
+int mf_level_set_state(void *level, const double *state, int is_device);
+int mf_level_get_state(void *level, double *state, int is_device);
+/* Device pointer of the current state (padded by MF_NGROW; strides returned). */
+int mf_level_state_view(void *level, mf_array4 *view);
+
+/* FillPatchSingleLevel of the current state at `time` (Source/mflo_gridwork.cpp:135-146,
+ * Source/mflo_bcfill.H:16-128): periodic images, foextrap, wall reflection and
+ * the plugin's externalbc, written into the ghost layers of the level array. */
+int mf_fill_ghosts_single_level(void *level, double time);
+
+/* EstTimeStep + ComputeDt (Source/mflo_timestep.cpp:18-153) on the current state. */
+int mf_est_dt(void *level, double *dt_out);
+
+/* EvolveAMR single-level step loop (Source/mflo_evolve.cpp:18-89): nsteps
+ * coarse steps, each ComputeDt -> Advance_coupled_strang (flow RK3 + explicit
+ * chemistry) -> update_primitive_vars -> clip_neg_speciesconc ->
+ * compute_residual_norms.  mode selects how the same kernels are launched:
+ *   MF_MODE_PER_BOX   one launch per box and kernel (the reference's shape),
+ *   MF_MODE_BATCHED   one persistent launch per kernel over the box/tile list,
+ *   MF_MODE_GRAPH     MF_MODE_BATCHED with each step replayed from a hipGraph. */
+enum { MF_MODE_PER_BOX = 0, MF_MODE_BATCHED = 1, MF_MODE_GRAPH = 2 };
+int mf_level_step(void *level, int nsteps, int mode);
+
+/* Time, last dt, step count and residual norms (TOTAL_NVARS doubles, may be NULL). */
+int mf_level_info(void *level, double *time, double *dt, int *istep, double *resnorms);
+/* Overrides the level time/dt history (restart). */
+int mf_level_set_time(void *level, double time, double dt_prev, int istep);
+
+/* Tile shape used by the level kernels (0 = autotune on the next step). */
+int mf_level_set_tile(void *level, int tx, int ty, int tz);
+int mf_level_get_tile(void *level, int *tx, int *ty, int *tz);
+/* Runs the autotuner now on the current state (state is restored afterwards). */
+int mf_level_autotune(void *level);
+
+/* ------------------------------------------------------------------------ */
+/* Discrete adjoint (no counterpart in the reference)                         */
+/* ------------------------------------------------------------------------ */
+/* Objective J = sum_n dt * (outlet species mass flux through x-hi of species
+ * `spec`), accumulated over nsteps fixed-dt steps. Forward stores binomial
+ * checkpoints (ncheck states); backward returns dJ/d(log_A[r]) for r < nreact
+ * and dJ/d(in_vel). */
+int mf_adjoint_forward(void *level, int nsteps, int ncheck, int spec, double *J_out);
+int mf_adjoint_backward(void *level, double *grad_logA, double *grad_invel);
+
+/* Plain HBM copy of n doubles on the context stream (bandwidth calibration). */
+int mf_util_copy(void *ctx, double *dst, const double *src, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MESOFLOW_AMD_H */

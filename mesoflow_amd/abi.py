@@ -1,0 +1,115 @@
+"""ctypes mirror of include/mesoflow_amd.h (structs, enums, status codes).
+
+The C-ABI is the product boundary; this module only describes its types so that
+Python callers (tests, bench, the autograd wrapper) can build the arguments.
+"""
+import ctypes as C
+
+MF_MAX_SPECIES = 16
+MF_MAX_REACTIONS = 8
+MF_NGROW = 3
+
+# component indices, Source/globalDefines.H:19-32
+RHO, RHOU, RHOV, RHOW, RHOE = 0, 1, 2, 3, 4
+DENS, VELX, VELY, VELZ, PRES, TEMP, VFRAC = 5, 6, 7, 8, 9, 10, 11
+FLO_NVARS = 12
+NCVARS = 5
+
+MF_OK = 0
+MF_ERR_NAN_DSDT = 1
+MF_ERR_NEG_PRESSURE = 2
+MF_ERR_UNKNOWN_SCHEME = 3
+MF_ERR_BAD_BC = 4
+MF_ERR_BAD_ARG = 5
+MF_ERR_HIP = 6
+MF_ERR_UNSUPPORTED = 7
+MF_ERR_NO_DEVICE = 8
+MF_ERR_TOO_LARGE = 9
+
+STATUS_NAMES = {
+    0: "MF_OK", 1: "MF_ERR_NAN_DSDT", 2: "MF_ERR_NEG_PRESSURE", 3: "MF_ERR_UNKNOWN_SCHEME",
+    4: "MF_ERR_BAD_BC", 5: "MF_ERR_BAD_ARG", 6: "MF_ERR_HIP", 7: "MF_ERR_UNSUPPORTED",
+    8: "MF_ERR_NO_DEVICE", 9: "MF_ERR_TOO_LARGE",
+}
+
+BC_PERIODIC, BC_FOEXTRAP, BC_EXT_DIR, BC_WALL = 0, 2, 3, 6
+
+PLUGIN_SHOCKTUBE_N2 = 1
+PLUGIN_CATALYST_CYLINDER = 2
+PLUGIN_POROUS_ARRHENIUS = 3
+
+MODE_PER_BOX, MODE_BATCHED, MODE_GRAPH = 0, 1, 2
+
+
+class Array4(C.Structure):
+    _fields_ = [("p", C.c_void_p), ("jstride", C.c_int64), ("kstride", C.c_int64),
+                ("nstride", C.c_int64), ("begin", C.c_int32 * 3), ("end", C.c_int32 * 3),
+                ("ncomp", C.c_int32), ("pad_", C.c_int32)]
+
+
+class Box(C.Structure):
+    _fields_ = [("lo", C.c_int32 * 3), ("hi", C.c_int32 * 3)]
+
+
+class Geom(C.Structure):
+    _fields_ = [("prob_lo", C.c_double * 3), ("prob_hi", C.c_double * 3),
+                ("n_cell", C.c_int32 * 3), ("is_periodic", C.c_int32 * 3)]
+
+
+class Params(C.Structure):
+    _fields_ = [("order_hyp", C.c_int32), ("do_ns", C.c_int32), ("species_in_solid", C.c_int32),
+                ("conj_ht", C.c_int32), ("using_bg_inertgas", C.c_int32), ("clip_species", C.c_int32),
+                ("track_residual_norms", C.c_int32), ("do_chemistry", C.c_int32),
+                ("dissfactor", C.c_double), ("cfl", C.c_double), ("stop_time", C.c_double),
+                ("lo_bc", C.c_int32 * 3), ("hi_bc", C.c_int32 * 3),
+                ("fixed_dt", C.c_int32), ("pad_", C.c_int32), ("dt_fixed", C.c_double)]
+
+
+_S = C.c_double * MF_MAX_SPECIES
+
+
+class PluginParams(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("nspec", C.c_int32), ("ngas", C.c_int32), ("nreact", C.c_int32),
+                ("R_univ", C.c_double), ("molwt", _S), ("gamma", _S), ("advect_flag", _S),
+                ("visc", C.c_double), ("thcond", C.c_double), ("dcoef_gas", _S), ("dcoef_solid", _S),
+                ("stoich", (C.c_double * MF_MAX_SPECIES) * MF_MAX_REACTIONS),
+                ("log_A", C.c_double * MF_MAX_REACTIONS), ("Ea", C.c_double * MF_MAX_REACTIONS),
+                ("kconst", C.c_double),
+                ("left_p", C.c_double), ("left_rho", C.c_double), ("left_T", C.c_double),
+                ("right_p", C.c_double), ("right_rho", C.c_double), ("right_T", C.c_double),
+                ("left_spec", _S), ("right_spec", _S),
+                ("in_vel", C.c_double), ("in_temp", C.c_double), ("in_rho", C.c_double), ("in_pres", C.c_double),
+                ("in_spec", _S), ("out_pres", C.c_double), ("siteconc", C.c_double)]
+
+
+def make_box(lo, hi):
+    b = Box()
+    for d in range(3):
+        b.lo[d] = int(lo[d])
+        b.hi[d] = int(hi[d])
+    return b
+
+
+def make_array4(ptr, begin, end, ncomp):
+    """Dense component-major view of the box [begin, end) (end exclusive)."""
+    a = Array4()
+    a.p = ptr
+    nx, ny, nz = (end[0] - begin[0]), (end[1] - begin[1]), (end[2] - begin[2])
+    a.jstride = nx
+    a.kstride = nx * ny
+    a.nstride = nx * ny * nz
+    for d in range(3):
+        a.begin[d] = int(begin[d])
+        a.end[d] = int(end[d])
+    a.ncomp = int(ncomp)
+    return a
+
+
+def status_name(st):
+    return STATUS_NAMES.get(int(st), "MF_ERR_%d" % int(st))
+
+
+class MFError(RuntimeError):
+    def __init__(self, where, st):
+        super().__init__("%s failed: %s" % (where, status_name(st)))
+        self.status = int(st)

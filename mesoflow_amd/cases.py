@@ -1,0 +1,442 @@
+"""Problem set-ups: the inputs files, plugin tables and initdata of the cases.
+
+Each builder returns a Case: the plugin table (mf_plugin_params), the runtime
+parameters (mf_params), the geometry, and a function that makes the initial
+state as a numpy array of shape (NV, nz, ny, nx) (component-major, x fastest,
+the layout of the C-ABI).
+
+ * shocktube_n2     tests/Cartesian/ShockTube_N2 (inputs_x, Prob.H:13-81,
+                    userfuncs.cpp:3-37; R_univ stays 1.0 because the case reads
+                    prob.univr while its inputs set prob.R_univ)
+ * catalyst_cylinder models/catalystCylinder (inputs, Prob.H:13-129,
+                    userfuncs.cpp:6-26, species.cpp:9-23)
+ * porous_reactor   a packed bed of seeded spheres or a thresholded Gaussian
+                    random field, NS gas+site species with Arrhenius kinetics,
+                    inflow at x-lo (u, T, composition), outflow foextrap at
+                    x-hi, periodic y and z; volume fractions from the 2x2x2
+                    corner sampling of models/catalystCylinder/Prob.H:41-59.
+"""
+import math
+from dataclasses import dataclass, field
+from typing import Callable, List, Tuple
+
+import numpy as np
+
+from . import abi
+
+RU = 8.31446261815324   # Source/mflo_constants.H:23
+
+
+@dataclass
+class Case:
+    name: str
+    plugin: abi.PluginParams
+    params: abi.Params
+    geom: abi.Geom
+    make_state: Callable[[], np.ndarray]
+    boxes: List[Tuple[Tuple[int, int, int], Tuple[int, int, int]]] = field(default_factory=list)
+    nsteps: int = 1
+
+    @property
+    def nspec(self):
+        return int(self.plugin.nspec)
+
+    @property
+    def nv(self):
+        return abi.FLO_NVARS + self.nspec
+
+    @property
+    def n_cell(self):
+        return tuple(int(self.geom.n_cell[d]) for d in range(3))
+
+    @property
+    def dx(self):
+        return tuple((self.geom.prob_hi[d] - self.geom.prob_lo[d]) / self.geom.n_cell[d] for d in range(3))
+
+    def box_structs(self):
+        bl = self.boxes or [((0, 0, 0), tuple(n - 1 for n in self.n_cell))]
+        arr = (abi.Box * len(bl))()
+        for i, (lo, hi) in enumerate(bl):
+            arr[i] = abi.make_box(lo, hi)
+        return arr, len(bl)
+
+
+def default_params(order_hyp=2, cfl=0.7, lo_bc=(2, 2, 2), hi_bc=(2, 2, 2)):
+    """mflo.* defaults of Source/mflo.H:226-259."""
+    p = abi.Params()
+    p.order_hyp = order_hyp
+    p.do_ns = 1
+    p.species_in_solid = 1
+    p.conj_ht = 0
+    p.using_bg_inertgas = 0
+    p.clip_species = 1
+    p.track_residual_norms = 1
+    p.do_chemistry = 1
+    p.dissfactor = 0.5
+    p.cfl = cfl
+    p.stop_time = 1.0e300
+    for d in range(3):
+        p.lo_bc[d] = lo_bc[d]
+        p.hi_bc[d] = hi_bc[d]
+    p.fixed_dt = 0
+    p.dt_fixed = 0.0
+    return p
+
+
+def make_geom(prob_lo, prob_hi, n_cell, periodic):
+    g = abi.Geom()
+    for d in range(3):
+        g.prob_lo[d] = prob_lo[d]
+        g.prob_hi[d] = prob_hi[d]
+        g.n_cell[d] = n_cell[d]
+        g.is_periodic[d] = int(periodic[d])
+    return g
+
+
+def chop_boxes(n_cell, max_grid_size):
+    """Uniform chopping of the domain into boxes of at most max_grid_size cells per edge."""
+    cuts = []
+    for d in range(3):
+        m = max(1, int(max_grid_size[d] if hasattr(max_grid_size, "__len__") else max_grid_size))
+        edges = list(range(0, n_cell[d], m)) + [n_cell[d]]
+        cuts.append([(edges[i], edges[i + 1] - 1) for i in range(len(edges) - 1)])
+    boxes = []
+    for (zl, zh) in cuts[2]:
+        for (yl, yh) in cuts[1]:
+            for (xl, xh) in cuts[0]:
+                boxes.append(((xl, yl, zl), (xh, yh, zh)))
+    return boxes
+
+
+def random_boxes(n_cell, edge_lo, edge_hi, seed):
+    """Uneven box decomposition: recursive bisection at random positions until
+    every edge is within [edge_lo, edge_hi] (boxes still tile the domain)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    stack = [((0, 0, 0), tuple(n - 1 for n in n_cell))]
+    while stack:
+        lo, hi = stack.pop()
+        ext = [hi[d] - lo[d] + 1 for d in range(3)]
+        big = [d for d in range(3) if ext[d] > edge_hi]
+        cand = [d for d in range(3) if ext[d] >= 2 * edge_lo]
+        if not big and (not cand or rng.random() < 0.35):
+            out.append((lo, hi))
+            continue
+        d = big[int(rng.integers(len(big)))] if big else cand[int(rng.integers(len(cand)))]
+        lo_cut = lo[d] + edge_lo
+        hi_cut = hi[d] - edge_lo + 1
+        cut = int(rng.integers(lo_cut, hi_cut + 1)) if hi_cut > lo_cut else lo_cut
+        h1 = list(hi); h1[d] = cut - 1
+        l2 = list(lo); l2[d] = cut
+        stack.append((lo, tuple(h1)))
+        stack.append((tuple(l2), hi))
+    out.sort(key=lambda b: (b[0][2], b[0][1], b[0][0]))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# ShockTube_N2
+# ---------------------------------------------------------------------------
+def shocktube_plugin(R_univ=1.0, p_l=1.0, rho_l=1.0, p_r=0.1, rho_r=0.125):
+    t = abi.PluginParams()
+    t.kind = abi.PLUGIN_SHOCKTUBE_N2
+    t.nspec = 1
+    t.ngas = 1
+    t.nreact = 0
+    t.R_univ = R_univ
+    t.molwt[0] = 0.028
+    t.gamma[0] = 1.4
+    t.advect_flag[0] = 1.0
+    t.left_p, t.left_rho = p_l, rho_l
+    t.right_p, t.right_rho = p_r, rho_r
+    t.left_spec[0] = rho_l / t.molwt[0]
+    t.right_spec[0] = rho_r / t.molwt[0]
+    t.left_T = p_l / (t.left_spec[0] * R_univ)      # get_t_from_rpc
+    t.right_T = p_r / (t.right_spec[0] * R_univ)
+    return t
+
+
+def _shocktube_e(t, p, spec):
+    """get_e_from_rpc of tests/Cartesian/ShockTube_N2/thermo.H:110-132."""
+    cv = t.R_univ / (t.gamma[0] - 1.0) / t.molwt[0]
+    x0 = spec / spec
+    temp = p / (spec * t.R_univ)
+    return temp * (cv * x0)
+
+
+def shocktube_n2(n_cell=(64, 64, 64), prob_hi=(1.0, 1.0, 1.0), order_hyp=5, cfl=0.5, direction=0,
+                 max_grid_size=None, R_univ=1.0, nsteps=100):
+    """Sod problem along `direction`: x_dir < 0.5 holds the left state.
+
+    Boundaries as tests/Cartesian/ShockTube_N2/inputs_{x,y,z}: ext_dir at the
+    low end of the shock direction, wall at its high end, periodic across."""
+    t = shocktube_plugin(R_univ=R_univ)
+    lo_bc = [0, 0, 0]
+    hi_bc = [0, 0, 0]
+    lo_bc[direction] = abi.BC_EXT_DIR
+    hi_bc[direction] = abi.BC_WALL
+    periodic = [1, 1, 1]
+    periodic[direction] = 0
+    p = default_params(order_hyp=order_hyp, cfl=cfl, lo_bc=lo_bc, hi_bc=hi_bc)
+    g = make_geom((0.0, 0.0, 0.0), prob_hi, n_cell, periodic)
+    nx, ny, nz = n_cell
+    dx = [prob_hi[d] / n_cell[d] for d in range(3)]
+
+    def make_state():
+        nv = abi.FLO_NVARS + 1
+        s = np.zeros((nv, nz, ny, nx))
+        coords = [(np.arange(n_cell[d]) + 0.5) * dx[d] for d in range(3)]
+        xs = coords[direction]
+        left = xs < 0.5
+        shape = [1, 1, 1]
+        shape[2 - direction] = n_cell[direction]
+        left = left.reshape(shape)
+        left = np.broadcast_to(left, (nz, ny, nx))
+        pres = np.where(left, t.left_p, t.right_p)
+        dens = np.where(left, t.left_rho, t.right_rho)
+        temp = np.where(left, t.left_T, t.right_T)
+        spec = np.where(left, t.left_spec[0], t.right_spec[0])
+        s[abi.VFRAC] = 1.0
+        s[abi.PRES] = pres
+        s[abi.DENS] = dens
+        s[abi.TEMP] = temp
+        s[abi.FLO_NVARS] = spec
+        s[abi.RHO] = dens
+        s[abi.RHOE] = dens * _shocktube_e(t, pres, spec)
+        return s
+
+    boxes = chop_boxes(n_cell, max_grid_size) if max_grid_size else []
+    return Case("shocktube_n2", t, p, g, make_state, boxes, nsteps)
+
+
+# ---------------------------------------------------------------------------
+# Mixture thermo helpers (models/catalystCylinder/thermo.H:36-156)
+# ---------------------------------------------------------------------------
+def _mix_e(t, rho, p, spec):
+    ng = t.ngas
+    sum_ci = sum(spec[i] for i in range(ng))
+    temp = p / (sum_ci * t.R_univ)
+    cvc = sum((t.R_univ / (t.gamma[i] - 1.0)) * spec[i] for i in range(ng))
+    return cvc * temp / rho
+
+
+def _mix_T(t, p, spec):
+    sum_ci = sum(spec[i] for i in range(t.ngas))
+    return p / (sum_ci * t.R_univ)
+
+
+# ---------------------------------------------------------------------------
+# catalystCylinder
+# ---------------------------------------------------------------------------
+def catalyst_plugin(fs_vel=10.0, rho0=1.16, p0=1e5, Aconc_inlet=2.0, siteconc=1.0):
+    t = abi.PluginParams()
+    t.kind = abi.PLUGIN_CATALYST_CYLINDER
+    t.nspec = 4
+    t.ngas = 3
+    t.nreact = 1
+    t.R_univ = RU
+    for i, (m, g, a) in enumerate([(0.0289, 1.4, 1.0), (0.044, 1.2, 1.0), (0.044, 1.2, 1.0), (0.001, 1.0, 0.0)]):
+        t.molwt[i], t.gamma[i], t.advect_flag[i] = m, g, a
+    t.visc, t.thcond = 1e-5, 0.01
+    for i, (dg, ds) in enumerate([(1e-5, 1e-6), (1e-5, 1e-6), (1e-4, 1e-5), (0.0, 0.0)]):
+        t.dcoef_gas[i], t.dcoef_solid[i] = dg, ds
+    t.kconst = 20.0
+    t.in_vel, t.in_rho = fs_vel, rho0
+    t.in_spec[1] = Aconc_inlet
+    t.out_pres = p0
+    t.siteconc = siteconc
+    return t
+
+
+def catalyst_cylinder(n_cell=(256, 256, 4), order_hyp=5, cfl=0.3, cylrad=0.25e-3, fs_vel=10.0, rho0=1.16,
+                      p0=1e5, Aconc_inlet=2.0, Aconc_initial=2.0, siteconc=1.0, max_grid_size=None, nsteps=20):
+    t = catalyst_plugin(fs_vel, rho0, p0, Aconc_inlet, siteconc)
+    p = default_params(order_hyp=order_hyp, cfl=cfl, lo_bc=(3, 2, 0), hi_bc=(2, 2, 0))
+    prob_lo = (-6.0e-3, -15.0e-3, -0.375e-3)
+    prob_hi = (24.0e-3, 15.0e-3, 0.375e-3)
+    g = make_geom(prob_lo, prob_hi, n_cell, (0, 0, 1))
+    nx, ny, nz = n_cell
+    dx = [(prob_hi[d] - prob_lo[d]) / n_cell[d] for d in range(3)]
+
+    def make_state():
+        nv = abi.FLO_NVARS + 4
+        s = np.zeros((nv, nz, ny, nx))
+        # corner sampling (Prob.H:41-59)
+        xc = prob_lo[0] + np.arange(nx + 1) * dx[0]
+        yc = prob_lo[1] + np.arange(ny + 1) * dx[1]
+        outside = (xc[None, :] ** 2 + yc[:, None] ** 2) > cylrad * cylrad      # (ny+1, nx+1)
+        o = outside.astype(np.float64)
+        vf2 = (o[:-1, :-1] + o[:-1, 1:] + o[1:, :-1] + o[1:, 1:]) * 2.0            # z corners identical
+        vf = np.broadcast_to(vf2 * 0.125, (nz, ny, nx)).copy()
+        s[abi.VFRAC] = vf
+        s[abi.VELX] = np.where(vf < 1.0, 0.0, fs_vel)
+        s[abi.PRES] = p0
+        s[abi.DENS] = rho0
+        m = [t.molwt[i] for i in range(4)]
+        spec = np.zeros((4, nz, ny, nx))
+        spec[0] = (rho0 - Aconc_initial * m[1]) / m[0]
+        spec[1] = Aconc_initial
+        spec[3] = (1.0 - vf) * siteconc
+        dummy = np.zeros_like(spec)
+        dummy[0] = spec[0]
+        dummy[3] = spec[3]
+        s[abi.RHO] = rho0
+        s[abi.RHOU] = rho0 * s[abi.VELX]
+        Tm = np.where(vf == 1.0, _mix_T(t, p0, spec), _mix_T(t, p0, dummy))
+        s[abi.TEMP] = Tm
+        e = np.where(vf == 1.0, _mix_e(t, rho0, p0, spec), _mix_e(t, rho0, p0, dummy))
+        s[abi.RHOE] = rho0 * e + 0.5 * rho0 * s[abi.VELX] ** 2
+        use = np.where(vf > 0.0, spec, dummy)
+        for i in range(4):
+            s[abi.FLO_NVARS + i] = use[i]
+        return s
+
+    boxes = chop_boxes(n_cell, max_grid_size) if max_grid_size else []
+    return Case("catalyst_cylinder", t, p, g, make_state, boxes, nsteps)
+
+
+# ---------------------------------------------------------------------------
+# Porous reactor with Arrhenius kinetics (build-authored plugin)
+# ---------------------------------------------------------------------------
+def porous_plugin(nspec=5, nreact=2, seed=20261203, in_vel=1.0, in_temp=700.0, in_pres=1.0e5, siteconc=10.0):
+    """NS-1 gas species (0 inert carrier, 1..NS-2 reacting) + 1 site species.
+
+    Reaction r converts gas species a = 1 + r % (NS-2) into b = 1 + (r+1) % (NS-2);
+    even r are surface reactions that also consume a site (rate k c_a c_S),
+    odd r are gas-phase (rate k c_a). A ~ logU[1e3,1e8], Ea ~ U[50,150] kJ/mol."""
+    rng = np.random.default_rng(seed)
+    t = abi.PluginParams()
+    t.kind = abi.PLUGIN_POROUS_ARRHENIUS
+    t.nspec = nspec
+    t.ngas = nspec - 1
+    t.nreact = nreact
+    t.R_univ = RU
+    ng = nspec - 1
+    for i in range(ng):
+        t.molwt[i] = 0.0289 if i == 0 else 0.030 + 0.004 * i
+        t.gamma[i] = 1.4 if i == 0 else 1.3 - 0.01 * i
+        t.advect_flag[i] = 1.0
+        t.dcoef_gas[i] = 2.0e-5 + 2.0e-6 * i
+        t.dcoef_solid[i] = 2.0e-7
+    t.molwt[ng] = 0.001
+    t.gamma[ng] = 1.0
+    t.advect_flag[ng] = 0.0
+    t.dcoef_gas[ng] = 0.0
+    t.dcoef_solid[ng] = 0.0
+    t.visc, t.thcond = 3.0e-5, 0.05
+    nrg = nspec - 2
+    for r in range(nreact):
+        a = 1 + r % nrg
+        b = 1 + (r + 1) % nrg
+        t.stoich[r][a] = -1.0
+        t.stoich[r][b] += 1.0
+        if r % 2 == 0:
+            t.stoich[r][ng] = -1.0
+        t.log_A[r] = math.log(10.0 ** rng.uniform(3.0, 8.0))
+        t.Ea[r] = rng.uniform(50.0e3, 150.0e3)
+    t.in_vel, t.in_temp, t.in_pres = in_vel, in_temp, in_pres
+    # inflow mole fractions: carrier 0.8, reactant 1 takes 0.2
+    t.in_spec[0] = 0.8
+    t.in_spec[1] = 0.2
+    t.out_pres = in_pres
+    t.siteconc = siteconc
+    return t
+
+
+def sphere_pack_solid(n_cell, porosity, rmin, rmax, seed):
+    """Boolean model of overlapping spheres (radii U[rmin, rmax] cells) on the
+    (n+1)^3 corner grid, added until the solid corner fraction reaches 1-porosity."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = n_cell
+    solid = np.zeros((nz + 1, ny + 1, nx + 1), dtype=bool)
+    target = 1.0 - porosity
+    total = solid.size
+    count = 0
+    batch = 16
+    while count / total < target:
+        for _ in range(batch):
+            c = rng.uniform(0, 1, 3) * np.array([nx, ny, nz])
+            r = rng.uniform(rmin, rmax)
+            lo = [max(0, int(math.floor(c[d] - r))) for d in range(3)]
+            hi = [min(n_cell[d], int(math.ceil(c[d] + r))) for d in range(3)]
+            zz = np.arange(lo[2], hi[2] + 1)[:, None, None]
+            yy = np.arange(lo[1], hi[1] + 1)[None, :, None]
+            xx = np.arange(lo[0], hi[0] + 1)[None, None, :]
+            inside = (xx - c[0]) ** 2 + (yy - c[1]) ** 2 + (zz - c[2]) ** 2 <= r * r
+            sub = solid[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
+            sub |= inside
+        count = int(solid.sum())
+        batch = max(1, min(256, int(batch * 1.5)))
+    return solid
+
+
+def grf_solid(n_cell, porosity, corr_len, seed):
+    """Gaussian random field on the corner grid (Gaussian spectrum with
+    correlation length corr_len cells), solid where it exceeds its porosity quantile."""
+    rng = np.random.default_rng(seed)
+    shp = tuple(n + 1 for n in n_cell[::-1])
+    w = rng.standard_normal(shp)
+    kz = np.fft.fftfreq(shp[0])[:, None, None]
+    ky = np.fft.fftfreq(shp[1])[None, :, None]
+    kx = np.fft.rfftfreq(shp[2])[None, None, :]
+    k2 = (2 * np.pi) ** 2 * (kx ** 2 + ky ** 2 + kz ** 2)
+    f = np.fft.irfftn(np.fft.rfftn(w) * np.exp(-0.5 * k2 * corr_len ** 2), s=shp)
+    thr = np.quantile(f, porosity)
+    return f > thr
+
+
+def vfrac_from_corners(solid):
+    """2x2x2 corner sampling (models/catalystCylinder/Prob.H:41-59): vfrac = (#fluid corners)/8."""
+    g = (~solid).astype(np.float64)
+    s = (g[:-1, :-1, :-1] + g[:-1, :-1, 1:] + g[:-1, 1:, :-1] + g[:-1, 1:, 1:] +
+         g[1:, :-1, :-1] + g[1:, :-1, 1:] + g[1:, 1:, :-1] + g[1:, 1:, 1:])
+    return s * 0.125
+
+
+def porous_reactor(n_cell=(256, 256, 256), nspec=5, nreact=2, geometry="spheres", porosity=0.6, seed=20261203,
+                   dx=4.0e-6, order_hyp=5, cfl=0.5, rmin=6.0, rmax=14.0, corr_len=8.0, max_grid_size=None,
+                   boxes=None, in_vel=1.0, nsteps=50):
+    t = porous_plugin(nspec=nspec, nreact=nreact, seed=seed, in_vel=in_vel)
+    p = default_params(order_hyp=order_hyp, cfl=cfl, lo_bc=(3, 0, 0), hi_bc=(2, 0, 0))
+    prob_hi = tuple(dx * n for n in n_cell)
+    g = make_geom((0.0, 0.0, 0.0), prob_hi, n_cell, (0, 1, 1))
+    nx, ny, nz = n_cell
+    ng = nspec - 1
+
+    def make_state():
+        if geometry == "spheres":
+            solid = sphere_pack_solid(n_cell, porosity, rmin, rmax, seed + 1)
+        elif geometry == "grf":
+            solid = grf_solid(n_cell, porosity, corr_len, seed + 1)
+        elif geometry == "none":
+            solid = np.zeros((nz + 1, ny + 1, nx + 1), dtype=bool)
+        else:
+            raise ValueError(geometry)
+        vf = vfrac_from_corners(solid)
+        del solid
+        nv = abi.FLO_NVARS + nspec
+        s = np.zeros((nv, nz, ny, nx))
+        s[abi.VFRAC] = vf
+        ctot = t.in_pres / (t.R_univ * t.in_temp)
+        gas = np.array([t.in_spec[i] * ctot for i in range(ng)])
+        dummy = np.zeros(ng)
+        dummy[0] = ctot
+        pos = vf > 0.0
+        for i in range(ng):
+            s[abi.FLO_NVARS + i] = np.where(pos, gas[i], dummy[i])
+        s[abi.FLO_NVARS + ng] = (1.0 - vf) * t.siteconc
+        sp = s[abi.FLO_NVARS:abi.FLO_NVARS + ng]
+        dens = np.zeros((nz, ny, nx))
+        for i in range(ng):
+            dens += sp[i] * t.molwt[i]
+        s[abi.DENS] = dens
+        s[abi.PRES] = t.in_pres
+        s[abi.VELX] = np.where(vf < 1.0, 0.0, t.in_vel)
+        s[abi.TEMP] = _mix_T(t, t.in_pres, sp)
+        s[abi.RHO] = dens
+        s[abi.RHOU] = dens * s[abi.VELX]
+        s[abi.RHOE] = dens * _mix_e(t, dens, t.in_pres, sp) + 0.5 * dens * s[abi.VELX] ** 2
+        return s
+
+    bl = boxes if boxes is not None else (chop_boxes(n_cell, max_grid_size) if max_grid_size else [])
+    return Case("porous_reactor", t, p, g, make_state, bl, nsteps)

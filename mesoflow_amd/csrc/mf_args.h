@@ -1,0 +1,115 @@
+// mf_args.h -- argument blocks shared by the kernels and the host driver.
+#pragma once
+#include "mf_device.h"
+
+namespace mfd {
+
+constexpr int kMaxNV = FLO + MF_MAX_SPECIES;
+
+// Device-resident step state: everything a step needs from the previous one,
+// so that a whole step can be captured in a hipGraph and replayed.
+struct TimeState {
+    double t;          // t_new of the level (cur_time of EvolveAMR)
+    double dt;         // dt of the step in flight
+    double dt_prev;    // dt of the previous step (ComputeDt growth cap); 1e100 at start
+    double max_ws;     // norm0 of the wave speed (EstTimeStep)
+    double max_D;      // norm0 of the largest species diffusivity
+    double stop_time;
+    int istep;
+    int active;        // 0 once EvolveAMR's stop conditions are met
+    int err;           // first MF_ERR_* raised on the device
+    int have_est;      // max_ws/max_D are current
+    double resnorm[kMaxNV];
+};
+
+// One box of a level (or the single box of a per-box call).
+struct BoxDesc {
+    int lo[3], hi[3];
+    int ntile[3];
+    int tile_start;
+    View flux[3];      // face-centred flux of each direction; begin = lo
+};
+
+// Tiles [t0, t1) of the box/tile list (boxes == nullptr: the single box `one`).
+struct Work {
+    const BoxDesc *boxes;
+    int nboxes;
+    int t0, t1;
+    int tsh[3];
+    BoxDesc one;
+};
+
+struct StageArgs {
+    const Tab *tab;
+    View src;                  // stage source (padded level array or caller's Sborder)
+    const int *cutid;          // substitution of own-box cut cells (null: none)
+    const double *cutbuf;
+    int ncut;
+    int nx, ny;
+    double dx[3];
+    const TimeState *ts;       // null: use time_host / dt_host
+    double time_host, dt_host;
+    int stage;                 // 1..3: RHS time is t (stage 1) or t+dt (stages 2, 3)
+    int order;
+    double dissfactor;
+    int nsflag, spec_in_solid, conj_ht;
+    int flux_full;             // flux views hold all NV comps (else the 5+NS flux comps)
+    int *err;
+};
+
+struct ResArgs {
+    StageArgs st;
+    int combine;               // 0: write dsdt; 1: RK3 stage combine into dst
+    View dsdt;                 // combine == 0 (single box)
+    View old;                  // U^n (S_old)
+    View dst;                  // output state of the stage
+    int epilogue;              // stage 3: chemistry, primitives, clip, residual, dt partials
+    int do_chem, clip, track_res, do_ns;
+    double *partials;          // [tile][nv + 2]
+};
+
+struct FillArgs {
+    const Tab *tab;
+    View s;                    // padded level array
+    int n[3];
+    int periodic[3];
+    int lo_bc[3], hi_bc[3];
+    double prob_lo[3], dx[3];
+    const TimeState *ts;       // null: use time_host
+    double time_host;
+    int stage;
+};
+
+struct CutArgs {
+    const Tab *tab;
+    View s;
+    const int *list;           // [3][ncut] i,j,k of the cut cells
+    double *cutbuf;            // [nv][ncut]
+    int ncut;
+    double dx[3];
+    int spec_in_solid, conj_ht;
+    const TimeState *ts;
+    // dense per-box form (list == nullptr): cells of the box [lo, hi], result
+    // to tmp[nv][cells] and flag[cells]
+    int lo[3], hi[3];
+    double *tmp;
+    int *flag;
+};
+
+// Host launchers of one compiled plugin family.
+struct KernelSet {
+    int ns;
+    void (*fill)(const FillArgs &, int phase, hipStream_t);
+    void (*cut)(const CutArgs &, hipStream_t);
+    void (*cut_box)(const CutArgs &, hipStream_t);
+    void (*faces)(const StageArgs &, const Work &, int dir, int grid, int block, hipStream_t);
+    void (*residual)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
+    void (*est)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
+};
+
+extern const KernelSet ks_shocktube_n2;
+extern const KernelSet ks_catalyst_cylinder;
+extern const KernelSet ks_porous5;
+extern const KernelSet ks_porous10;
+
+}  // namespace mfd

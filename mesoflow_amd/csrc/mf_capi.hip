@@ -1,0 +1,1023 @@
+// mf_capi.hip -- host driver and C-ABI of the MI355X RHS + RK3 path.
+//
+// The level driver keeps three padded state arrays A, B, C per level:
+//   stage 1: A (U^n, ghost-filled)  -> B = S1
+//   stage 2: B (S1, ghost-filled)   -> C = S2
+//   stage 3: C (S2, ghost-filled)   -> A = S3, chemistry, primitives, clip,
+//            residual and dt partials fused (A is only read point-wise there)
+// so every kernel argument of a step is fixed and a step can be replayed from a
+// hipGraph. dt, time and step count live in a device TimeState.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+#include "mf_args.h"
+
+using namespace mfd;
+
+namespace {
+
+#define HIPCHK(x)                                      \
+    do {                                               \
+        hipError_t e_ = (x);                           \
+        if (e_ != hipSuccess) {                        \
+            last_hip_error = e_;                       \
+            return MF_ERR_HIP;                         \
+        }                                              \
+    } while (0)
+
+thread_local hipError_t last_hip_error = hipSuccess;
+
+struct Ctx {
+    mf_plugin_params tab;
+    mf_params prm;
+    mf_geom geom;
+    double dx[3];
+    int nv = 0, ns = 0;
+    Tab *d_tab = nullptr;
+    int *d_err = nullptr;
+    hipStream_t stream = nullptr;
+    const KernelSet *ks = nullptr;
+    int device = 0;
+    // scratch of the per-box C-ABI calls
+    double *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    int tsh[3] = {32, 4, 2};
+};
+
+struct Level {
+    Ctx *c = nullptr;
+    int n[3];
+    std::vector<mf_box> boxes;
+    std::vector<BoxDesc> hbox;
+    BoxDesc *d_box = nullptr;
+    int ntiles = 0;
+    int tsh[3] = {0, 0, 0};
+    int max_grid = 2048;
+    double *arr[3] = {nullptr, nullptr, nullptr};  // A, B, C
+    size_t padded = 0;
+    View view[3];
+    double *flux = nullptr;
+    size_t flux_elems = 0;
+    int *cutid = nullptr, *cutlist = nullptr;
+    double *cutbuf = nullptr;
+    int ncut = 0;
+    double *partials = nullptr;
+    size_t partials_cap = 0;
+    TimeState *ts = nullptr;
+    bool need_est = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
+
+const KernelSet *pick_kernels(const mf_plugin_params &t) {
+    switch (t.kind) {
+    case MF_PLUGIN_SHOCKTUBE_N2: return (t.nspec == 1 && t.ngas == 1) ? &ks_shocktube_n2 : nullptr;
+    case MF_PLUGIN_CATALYST_CYLINDER: return (t.nspec == 4 && t.ngas == 3) ? &ks_catalyst_cylinder : nullptr;
+    case MF_PLUGIN_POROUS_ARRHENIUS:
+        if (t.nspec == 5 && t.ngas == 4) return &ks_porous5;
+        if (t.nspec == 10 && t.ngas == 9) return &ks_porous10;
+        return nullptr;
+    }
+    return nullptr;
+}
+
+View padded_view(double *p, const int *n, int nv) {
+    View v;
+    v.p = p;
+    v.b0 = v.b1 = v.b2 = -MF_NGROW;
+    v.js = n[0] + 2 * MF_NGROW;
+    v.ks = v.js * (int64_t)(n[1] + 2 * MF_NGROW);
+    v.ns = v.ks * (int64_t)(n[2] + 2 * MF_NGROW);
+    (void)nv;
+    return v;
+}
+
+View from_array4(const mf_array4 &a) {
+    View v;
+    v.p = a.p;
+    v.js = a.jstride;
+    v.ks = a.kstride;
+    v.ns = a.nstride;
+    v.b0 = a.begin[0];
+    v.b1 = a.begin[1];
+    v.b2 = a.begin[2];
+    return v;
+}
+
+int div_up(int a, int b) { return (a + b - 1) / b; }
+
+// ---------------------------------------------------------------------------
+// plugin-independent kernels
+// ---------------------------------------------------------------------------
+__global__ void k_pack(const double *__restrict__ dense, View pad, int nx, int ny, int nz, int nv, int to_pad) {
+    const int64_t ncell = (int64_t)nx * ny * nz;
+    const int64_t total = ncell * nv;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        int64_t c = q / ncell, r = q % ncell;
+        int i = (int)(r % nx), j = (int)((r / nx) % ny), k = (int)(r / ((int64_t)nx * ny));
+        if (to_pad) pad(i, j, k, (int)c) = dense[q];
+        else const_cast<double *>(dense)[q] = pad(i, j, k, (int)c);
+    }
+}
+
+__global__ void k_combine(int stage, View snew, View src, View dsdt, int lo0, int lo1, int lo2, int ex, int ey, int ez, int nv,
+                          double dt) {
+    const int64_t ncell = (int64_t)ex * ey * ez;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ncell * nv; q += (int64_t)gridDim.x * blockDim.x) {
+        int n = (int)(q / ncell);
+        int64_t r = q % ncell;
+        int i = lo0 + (int)(r % ex), j = lo1 + (int)((r / ex) % ey), k = lo2 + (int)(r / ((int64_t)ex * ey));
+        double d = dsdt(i, j, k, n);
+        if (stage == 1) snew(i, j, k, n) = kOne * src(i, j, k, n) + dt * d;
+        else if (stage == 2) {
+            double v = snew(i, j, k, n);
+            v += dt * d;
+            v += 3.0 * src(i, j, k, n);
+            snew(i, j, k, n) = v * kFourth;
+        } else {
+            double v = snew(i, j, k, n);
+            v += dt * d;
+            v = src(i, j, k, n) + kTwo * v;
+            snew(i, j, k, n) = v * kThird;
+        }
+    }
+}
+
+__global__ void k_scatter_cut(View sb, const double *tmp, const int *flag, int lo0, int lo1, int lo2, int ex, int ey, int ez,
+                              int nv) {
+    const int64_t ncell = (int64_t)ex * ey * ez;
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= ncell || !flag[q]) return;
+    int i = lo0 + (int)(q % ex), j = lo1 + (int)((q / ex) % ey), k = lo2 + (int)(q / ((int64_t)ex * ey));
+    for (int n = 0; n < nv; ++n) sb(i, j, k, n) = tmp[(int64_t)n * ncell + q];
+}
+
+// ComputeDt (Source/mflo_timestep.cpp:18-50) + EstTimeStep's final formula (:132-152)
+__global__ void k_compute_dt(TimeState *ts, double dx0, double dx1, double dx2, double cfl, int fixed, double dt_fixed) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (!ts->active || ts->err) return;
+    if (!(ts->t < ts->stop_time)) { ts->active = 0; return; }
+    const double dx[3] = {dx0, dx1, dx2};
+    double est = DBL_MAX;
+    if (ts->max_ws > 0)
+        for (int d = 0; d < 3; ++d) est = fmin(est, dx[d] / ts->max_ws);
+    if (ts->max_D > 0)
+        for (int d = 0; d < 3; ++d) est = fmin(est, kHalf * dx[d] * dx[d] / ts->max_D / 3);
+    est *= cfl;
+    if (fixed) est = dt_fixed;
+    const double change_max = 1.1;
+    double dt0 = est;
+    double dtl = fmin(est, change_max * ts->dt_prev);
+    dt0 = fmin(dt0, 1 * dtl);
+    if (fixed) dt0 = dt_fixed;
+    const double eps = 1.e-3 * dt0;
+    if (ts->t + dt0 > ts->stop_time - eps) dt0 = ts->stop_time - ts->t;
+    ts->dt = dt0;
+}
+
+// reduces the per-tile partials; mode 0 closes a step, mode 1 only refreshes
+// the dt estimate inputs
+__global__ __launch_bounds__(256) void k_finalize(const double *partials, int ntiles, int nv, TimeState *ts, int *err,
+                                                  int mode, int track) {
+    __shared__ double red[256];
+    const int np = nv + 2;
+    if (mode == 0 && (!ts->active || ts->err)) {
+        if (threadIdx.x == 0 && *err && !ts->err) ts->err = *err;
+        return;
+    }
+    double vals[kMaxNV + 2];
+    for (int q = 0; q < np; ++q) {
+        const bool is_max = q >= nv;
+        double v = 0.0;
+        for (int t = threadIdx.x; t < ntiles; t += blockDim.x) {
+            double x = partials[(size_t)t * np + q];
+            v = is_max ? fmax(v, x) : v + x;
+        }
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+                double a = red[threadIdx.x], b = red[threadIdx.x + s];
+                red[threadIdx.x] = is_max ? fmax(a, b) : a + b;
+            }
+            __syncthreads();
+        }
+        vals[q] = red[0];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    ts->max_ws = vals[nv];
+    ts->max_D = vals[nv + 1];
+    ts->have_est = 1;
+    if (mode == 1) return;
+    if (track)
+        for (int q = 0; q < nv; ++q) ts->resnorm[q] = sqrt(vals[q]);
+    ts->t = ts->t + ts->dt;
+    ts->dt_prev = ts->dt;
+    ts->istep += 1;
+    if (*err) ts->err = *err;
+    if (ts->t >= ts->stop_time - 1.e-6 * ts->dt) ts->active = 0;
+}
+
+__global__ void k_copy(double *__restrict__ dst, const double *__restrict__ src, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// level helpers
+// ---------------------------------------------------------------------------
+StageArgs stage_args(Level *L, int stage) {
+    Ctx *c = L->c;
+    StageArgs a;
+    a.tab = c->d_tab;
+    a.src = L->view[stage == 1 ? 0 : (stage == 2 ? 1 : 2)];
+    a.cutid = L->ncut ? L->cutid : nullptr;
+    a.cutbuf = L->cutbuf;
+    a.ncut = L->ncut;
+    a.nx = L->n[0];
+    a.ny = L->n[1];
+    for (int d = 0; d < 3; ++d) a.dx[d] = c->dx[d];
+    a.ts = L->ts;
+    a.time_host = 0.0;
+    a.dt_host = 0.0;
+    a.stage = stage;
+    a.order = c->prm.order_hyp;
+    a.dissfactor = c->prm.dissfactor;
+    a.nsflag = c->prm.do_ns;
+    a.spec_in_solid = c->prm.species_in_solid;
+    a.conj_ht = c->prm.conj_ht;
+    a.flux_full = 0;
+    a.err = c->d_err;
+    return a;
+}
+
+Work level_work(Level *L) {
+    Work w;
+    w.boxes = L->d_box;
+    w.nboxes = (int)L->boxes.size();
+    w.t0 = 0;
+    w.t1 = L->ntiles;
+    for (int d = 0; d < 3; ++d) w.tsh[d] = L->tsh[d];
+    std::memset(&w.one, 0, sizeof(w.one));
+    return w;
+}
+
+FillArgs fill_args(Level *L, double *arr, int stage) {
+    Ctx *c = L->c;
+    FillArgs f;
+    f.tab = c->d_tab;
+    f.s = padded_view(arr, L->n, c->nv);
+    for (int d = 0; d < 3; ++d) {
+        f.n[d] = L->n[d];
+        f.periodic[d] = c->geom.is_periodic[d];
+        f.lo_bc[d] = c->prm.lo_bc[d];
+        f.hi_bc[d] = c->prm.hi_bc[d];
+        f.prob_lo[d] = c->geom.prob_lo[d];
+        f.dx[d] = c->dx[d];
+    }
+    f.ts = L->ts;
+    f.time_host = 0.0;
+    f.stage = stage;
+    return f;
+}
+
+CutArgs cut_args(Level *L, double *arr) {
+    Ctx *c = L->c;
+    CutArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.tab = c->d_tab;
+    a.s = padded_view(arr, L->n, c->nv);
+    a.list = L->cutlist;
+    a.cutbuf = L->cutbuf;
+    a.ncut = L->ncut;
+    for (int d = 0; d < 3; ++d) a.dx[d] = c->dx[d];
+    a.spec_in_solid = c->prm.species_in_solid;
+    a.conj_ht = c->prm.conj_ht;
+    a.ts = L->ts;
+    return a;
+}
+
+int rebuild_tiles(Level *L, const int *tsh) {
+    Ctx *c = L->c;
+    for (int d = 0; d < 3; ++d) L->tsh[d] = std::max(1, tsh[d]);
+    const int NC = NCV + c->ns;
+    int tiles = 0;
+    size_t fofs = 0;
+    L->hbox.resize(L->boxes.size());
+    for (size_t b = 0; b < L->boxes.size(); ++b) {
+        BoxDesc &bd = L->hbox[b];
+        std::memset(&bd, 0, sizeof(bd));
+        const mf_box &B = L->boxes[b];
+        int len[3];
+        for (int d = 0; d < 3; ++d) {
+            bd.lo[d] = B.lo[d];
+            bd.hi[d] = B.hi[d];
+            len[d] = B.hi[d] - B.lo[d] + 1;
+            bd.ntile[d] = div_up(len[d], L->tsh[d]);
+        }
+        bd.tile_start = tiles;
+        tiles += bd.ntile[0] * bd.ntile[1] * bd.ntile[2];
+        for (int d = 0; d < 3; ++d) {
+            int fl[3] = {len[0], len[1], len[2]};
+            fl[d] += 1;
+            View &v = bd.flux[d];
+            v.p = L->flux + fofs;
+            v.b0 = B.lo[0];
+            v.b1 = B.lo[1];
+            v.b2 = B.lo[2];
+            v.js = fl[0];
+            v.ks = (int64_t)fl[0] * fl[1];
+            v.ns = v.ks * fl[2];
+            fofs += (size_t)v.ns * NC;
+        }
+    }
+    L->ntiles = tiles;
+    if (L->d_box) hipFree(L->d_box);
+    HIPCHK(hipMalloc(&L->d_box, sizeof(BoxDesc) * L->hbox.size()));
+    HIPCHK(hipMemcpy(L->d_box, L->hbox.data(), sizeof(BoxDesc) * L->hbox.size(), hipMemcpyHostToDevice));
+    size_t need = (size_t)tiles * (c->nv + 2);
+    if (need > L->partials_cap) {
+        if (L->partials) hipFree(L->partials);
+        HIPCHK(hipMalloc(&L->partials, need * sizeof(double)));
+        L->partials_cap = need;
+    }
+    if (L->gexec) { hipGraphExecDestroy(L->gexec); L->gexec = nullptr; }
+    if (L->graph) { hipGraphDestroy(L->graph); L->graph = nullptr; }
+    return MF_OK;
+}
+
+int build_cutlist(Level *L, const double *vfrac_host) {
+    const int nx = L->n[0], ny = L->n[1], nz = L->n[2];
+    const int64_t ncell = (int64_t)nx * ny * nz;
+    std::vector<int> id(ncell, -1), li, lj, lk;
+    int nc = 0;
+    for (int64_t q = 0; q < ncell; ++q) {
+        double v = vfrac_host[q];
+        if (v > 0.0 && v < 1.0) {
+            id[q] = nc++;
+            li.push_back((int)(q % nx));
+            lj.push_back((int)((q / nx) % ny));
+            lk.push_back((int)(q / ((int64_t)nx * ny)));
+        }
+    }
+    if (L->cutlist) { hipFree(L->cutlist); L->cutlist = nullptr; }
+    if (L->cutbuf) { hipFree(L->cutbuf); L->cutbuf = nullptr; }
+    L->ncut = nc;
+    HIPCHK(hipMemcpy(L->cutid, id.data(), sizeof(int) * ncell, hipMemcpyHostToDevice));
+    if (nc) {
+        std::vector<int> lst;
+        lst.reserve(3 * (size_t)nc);
+        lst.insert(lst.end(), li.begin(), li.end());
+        lst.insert(lst.end(), lj.begin(), lj.end());
+        lst.insert(lst.end(), lk.begin(), lk.end());
+        HIPCHK(hipMalloc(&L->cutlist, sizeof(int) * lst.size()));
+        HIPCHK(hipMemcpy(L->cutlist, lst.data(), sizeof(int) * lst.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&L->cutbuf, sizeof(double) * (size_t)nc * L->c->nv));
+    }
+    if (L->gexec) { hipGraphExecDestroy(L->gexec); L->gexec = nullptr; }
+    if (L->graph) { hipGraphDestroy(L->graph); L->graph = nullptr; }
+    return MF_OK;
+}
+
+// launch of the face and residual kernels, one launch per box (mode 0) or one
+// persistent launch over the whole tile list
+void launch_faces(Level *L, const StageArgs &a, int dir, int mode) {
+    Ctx *c = L->c;
+    Work w = level_work(L);
+    if (mode == MF_MODE_PER_BOX) {
+        for (size_t b = 0; b < L->hbox.size(); ++b) {
+            const BoxDesc &bd = L->hbox[b];
+            w.t0 = bd.tile_start;
+            w.t1 = bd.tile_start + bd.ntile[0] * bd.ntile[1] * bd.ntile[2];
+            c->ks->faces(a, w, dir, w.t1 - w.t0, 256, c->stream);
+        }
+    } else {
+        c->ks->faces(a, w, dir, std::min(L->ntiles, L->max_grid), 256, c->stream);
+    }
+}
+void launch_residual(Level *L, const ResArgs &r, int mode) {
+    Ctx *c = L->c;
+    Work w = level_work(L);
+    if (mode == MF_MODE_PER_BOX) {
+        for (size_t b = 0; b < L->hbox.size(); ++b) {
+            const BoxDesc &bd = L->hbox[b];
+            w.t0 = bd.tile_start;
+            w.t1 = bd.tile_start + bd.ntile[0] * bd.ntile[1] * bd.ntile[2];
+            c->ks->residual(r, w, w.t1 - w.t0, 256, c->stream);
+        }
+    } else {
+        c->ks->residual(r, w, std::min(L->ntiles, L->max_grid), 256, c->stream);
+    }
+}
+
+void enqueue_stage(Level *L, int stage, int mode) {
+    Ctx *c = L->c;
+    double *src = L->arr[stage == 1 ? 0 : (stage == 2 ? 1 : 2)];
+    FillArgs f = fill_args(L, src, stage);
+    for (int ph = 0; ph <= 3; ++ph) c->ks->fill(f, ph, c->stream);
+    if (L->ncut) c->ks->cut(cut_args(L, src), c->stream);
+    StageArgs a = stage_args(L, stage);
+    for (int d = 0; d < 3; ++d) launch_faces(L, a, d, mode);
+    ResArgs r;
+    r.st = a;
+    r.combine = 1;
+    std::memset(&r.dsdt, 0, sizeof(r.dsdt));
+    r.old = L->view[0];
+    r.dst = L->view[stage == 1 ? 1 : (stage == 2 ? 2 : 0)];
+    r.epilogue = (stage == 3);
+    r.do_chem = c->prm.do_chemistry;
+    r.clip = c->prm.clip_species;
+    r.track_res = c->prm.track_residual_norms;
+    r.do_ns = c->prm.do_ns;
+    r.partials = L->partials;
+    launch_residual(L, r, mode);
+}
+
+void enqueue_step(Level *L, int mode) {
+    Ctx *c = L->c;
+    hipLaunchKernelGGL(k_compute_dt, dim3(1), dim3(1), 0, c->stream, L->ts, c->dx[0], c->dx[1], c->dx[2], c->prm.cfl,
+                       c->prm.fixed_dt, c->prm.dt_fixed);
+    for (int s = 1; s <= 3; ++s) enqueue_stage(L, s, mode);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, L->partials, L->ntiles, c->nv, L->ts, c->d_err, 0,
+                       c->prm.track_residual_norms);
+}
+
+void enqueue_est(Level *L) {
+    Ctx *c = L->c;
+    ResArgs r;
+    std::memset(&r, 0, sizeof(r));
+    r.st = stage_args(L, 1);
+    r.st.ts = nullptr;
+    r.do_ns = c->prm.do_ns;
+    r.partials = L->partials;
+    Work w = level_work(L);
+    c->ks->est(r, w, std::min(L->ntiles, L->max_grid), 256, c->stream);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, L->partials, L->ntiles, c->nv, L->ts, c->d_err, 1, 0);
+}
+
+int read_ts(Level *L, TimeState *h) {
+    HIPCHK(hipStreamSynchronize(L->c->stream));
+    HIPCHK(hipMemcpy(h, L->ts, sizeof(TimeState), hipMemcpyDeviceToHost));
+    return MF_OK;
+}
+
+int ensure_scratch(Ctx *c, size_t bytes) {
+    if (bytes <= c->scratch_bytes) return MF_OK;
+    if (c->scratch) { HIPCHK(hipStreamSynchronize(c->stream)); hipFree(c->scratch); c->scratch = nullptr; }
+    HIPCHK(hipMalloc(&c->scratch, bytes));
+    c->scratch_bytes = bytes;
+    return MF_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char *mf_status_string(int st) {
+    switch (st) {
+    case MF_OK: return "ok";
+    case MF_ERR_NAN_DSDT: return "NaN detected in dsdt";
+    case MF_ERR_NEG_PRESSURE: return "Pressure has gone negative";
+    case MF_ERR_UNKNOWN_SCHEME: return "unknown hyperbolics scheme";
+    case MF_ERR_BAD_BC: return "invalid boundary condition";
+    case MF_ERR_BAD_ARG: return "invalid argument";
+    case MF_ERR_HIP: return hipGetErrorString(last_hip_error);
+    case MF_ERR_UNSUPPORTED: return "plugin family / species count not compiled";
+    case MF_ERR_NO_DEVICE: return "no HIP device";
+    case MF_ERR_TOO_LARGE: return "size above the supported maximum";
+    }
+    return "unknown status";
+}
+
+int mf_device_count(int *count) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    if (count) *count = n;
+    return n > 0 ? MF_OK : MF_ERR_NO_DEVICE;
+}
+
+int mf_ctx_create(const mf_plugin_params *plugin, const mf_params *params, const mf_geom *geom, void **ctx_out) {
+    if (!plugin || !params || !geom || !ctx_out) return MF_ERR_BAD_ARG;
+    *ctx_out = nullptr;
+    const KernelSet *ks = pick_kernels(*plugin);
+    if (!ks) return MF_ERR_UNSUPPORTED;
+    int o = params->order_hyp;
+    if (!(o == 1 || o == 2 || o == 4 || o == 5)) return MF_ERR_UNKNOWN_SCHEME;
+    for (int d = 0; d < 3; ++d) {
+        if (geom->n_cell[d] <= 0 || !(geom->prob_hi[d] > geom->prob_lo[d])) return MF_ERR_BAD_ARG;
+        if (geom->is_periodic[d]) continue;
+        const int bcs[2] = {params->lo_bc[d], params->hi_bc[d]};
+        for (int s = 0; s < 2; ++s)
+            if (!(bcs[s] == MF_BC_FOEXTRAP || bcs[s] == MF_BC_EXT_DIR || bcs[s] == MF_BC_WALL)) return MF_ERR_BAD_BC;
+    }
+    if (plugin->nreact < 0 || plugin->nreact > MF_MAX_REACTIONS) return MF_ERR_BAD_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MF_ERR_NO_DEVICE;
+    Ctx *c = new (std::nothrow) Ctx();
+    if (!c) return MF_ERR_BAD_ARG;
+    c->tab = *plugin;
+    c->prm = *params;
+    c->geom = *geom;
+    c->ks = ks;
+    c->ns = plugin->nspec;
+    c->nv = FLO + c->ns;
+    for (int d = 0; d < 3; ++d) c->dx[d] = (geom->prob_hi[d] - geom->prob_lo[d]) / geom->n_cell[d];
+    hipGetDevice(&c->device);
+    if (hipMalloc(&c->d_tab, sizeof(Tab)) != hipSuccess || hipMalloc(&c->d_err, sizeof(int)) != hipSuccess ||
+        hipMemcpy(c->d_tab, plugin, sizeof(Tab), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->d_err, 0, sizeof(int)) != hipSuccess) {
+        delete c;
+        return MF_ERR_HIP;
+    }
+    *ctx_out = c;
+    return MF_OK;
+}
+
+int mf_ctx_destroy(void *ctx) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c) return MF_ERR_BAD_ARG;
+    hipStreamSynchronize(c->stream);
+    if (c->d_tab) hipFree(c->d_tab);
+    if (c->d_err) hipFree(c->d_err);
+    if (c->scratch) hipFree(c->scratch);
+    delete c;
+    return MF_OK;
+}
+
+int mf_ctx_set_stream(void *ctx, void *stream) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c) return MF_ERR_BAD_ARG;
+    c->stream = (hipStream_t)stream;
+    return MF_OK;
+}
+
+int mf_ctx_nvars(void *ctx, int *nv) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c || !nv) return MF_ERR_BAD_ARG;
+    *nv = c->nv;
+    return MF_OK;
+}
+
+int mf_ctx_sync(void *ctx) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c) return MF_ERR_BAD_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    int e = 0;
+    HIPCHK(hipMemcpy(&e, c->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (e) HIPCHK(hipMemset(c->d_err, 0, sizeof(int)));
+    return e;
+}
+
+// ----- per-box plain forms ---------------------------------------------------
+int mf_box_cutcells(void *ctx, mf_box bx, mf_array4 sborder, double time) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c || !sborder.p || sborder.ncomp < c->nv) return MF_ERR_BAD_ARG;
+    (void)time;
+    const int ex = bx.hi[0] - bx.lo[0] + 1, ey = bx.hi[1] - bx.lo[1] + 1, ez = bx.hi[2] - bx.lo[2] + 1;
+    if (ex <= 0 || ey <= 0 || ez <= 0) return MF_ERR_BAD_ARG;
+    for (int d = 0; d < 3; ++d)
+        if (bx.lo[d] - MF_NGROW < sborder.begin[d] || bx.hi[d] + MF_NGROW >= sborder.end[d]) return MF_ERR_BAD_ARG;
+    const int64_t ncell = (int64_t)ex * ey * ez;
+    int st = ensure_scratch(c, (size_t)ncell * (c->nv * sizeof(double) + sizeof(int)));
+    if (st) return st;
+    CutArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.tab = c->d_tab;
+    a.s = from_array4(sborder);
+    for (int d = 0; d < 3; ++d) { a.dx[d] = c->dx[d]; a.lo[d] = bx.lo[d]; a.hi[d] = bx.hi[d]; }
+    a.spec_in_solid = c->prm.species_in_solid;
+    a.conj_ht = c->prm.conj_ht;
+    a.tmp = c->scratch;
+    a.flag = (int *)(c->scratch + (size_t)ncell * c->nv);
+    c->ks->cut_box(a, c->stream);
+    hipLaunchKernelGGL(k_scatter_cut, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, c->stream, a.s, a.tmp, a.flag,
+                       bx.lo[0], bx.lo[1], bx.lo[2], ex, ey, ez, c->nv);
+    HIPCHK(hipGetLastError());
+    return MF_OK;
+}
+
+int mf_box_rhs(void *ctx, mf_box bx, mf_array4 sborder, mf_array4 dsdt, const mf_array4 *flux3, double time) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c || !sborder.p || !dsdt.p || sborder.ncomp < c->nv || dsdt.ncomp < c->nv) return MF_ERR_BAD_ARG;
+    int len[3];
+    for (int d = 0; d < 3; ++d) {
+        len[d] = bx.hi[d] - bx.lo[d] + 1;
+        if (len[d] <= 0) return MF_ERR_BAD_ARG;
+        if (bx.lo[d] - MF_NGROW < sborder.begin[d] || bx.hi[d] + MF_NGROW >= sborder.end[d]) return MF_ERR_BAD_ARG;
+        if (bx.lo[d] < dsdt.begin[d] || bx.hi[d] >= dsdt.end[d]) return MF_ERR_BAD_ARG;
+    }
+    Work w;
+    std::memset(&w, 0, sizeof(w));
+    w.boxes = nullptr;
+    for (int d = 0; d < 3; ++d) {
+        w.tsh[d] = c->tsh[d];
+        w.one.lo[d] = bx.lo[d];
+        w.one.hi[d] = bx.hi[d];
+        w.one.ntile[d] = div_up(len[d], w.tsh[d]);
+    }
+    w.t0 = 0;
+    w.t1 = w.one.ntile[0] * w.one.ntile[1] * w.one.ntile[2];
+    const int NC = NCV + c->ns;
+    int full = 0;
+    if (flux3) {
+        full = 1;
+        for (int d = 0; d < 3; ++d) {
+            if (!flux3[d].p || flux3[d].ncomp < c->nv) return MF_ERR_BAD_ARG;
+            w.one.flux[d] = from_array4(flux3[d]);
+        }
+    } else {
+        size_t tot = 0;
+        for (int d = 0; d < 3; ++d) {
+            int fl[3] = {len[0], len[1], len[2]};
+            fl[d] += 1;
+            tot += (size_t)fl[0] * fl[1] * fl[2] * NC;
+        }
+        int st = ensure_scratch(c, tot * sizeof(double));
+        if (st) return st;
+        size_t ofs = 0;
+        for (int d = 0; d < 3; ++d) {
+            int fl[3] = {len[0], len[1], len[2]};
+            fl[d] += 1;
+            View &v = w.one.flux[d];
+            v.p = c->scratch + ofs;
+            v.b0 = bx.lo[0]; v.b1 = bx.lo[1]; v.b2 = bx.lo[2];
+            v.js = fl[0];
+            v.ks = (int64_t)fl[0] * fl[1];
+            v.ns = v.ks * fl[2];
+            ofs += (size_t)v.ns * NC;
+        }
+    }
+    StageArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.tab = c->d_tab;
+    a.src = from_array4(sborder);
+    a.cutid = nullptr;
+    for (int d = 0; d < 3; ++d) a.dx[d] = c->dx[d];
+    a.ts = nullptr;
+    a.time_host = time;
+    a.stage = 1;
+    a.order = c->prm.order_hyp;
+    a.dissfactor = c->prm.dissfactor;
+    a.nsflag = c->prm.do_ns;
+    a.spec_in_solid = c->prm.species_in_solid;
+    a.conj_ht = c->prm.conj_ht;
+    a.flux_full = full;
+    a.err = c->d_err;
+    for (int d = 0; d < 3; ++d) c->ks->faces(a, w, d, w.t1, 256, c->stream);
+    ResArgs r;
+    std::memset(&r, 0, sizeof(r));
+    r.st = a;
+    r.combine = 0;
+    r.dsdt = from_array4(dsdt);
+    c->ks->residual(r, w, w.t1, 256, c->stream);
+    HIPCHK(hipGetLastError());
+    return MF_OK;
+}
+
+int mf_box_rk3_combine(void *ctx, int stage, mf_box bx, mf_array4 s_new, mf_array4 src, mf_array4 dsdt, double dt) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c || stage < 1 || stage > 3 || !s_new.p || !src.p || !dsdt.p) return MF_ERR_BAD_ARG;
+    const int ex = bx.hi[0] - bx.lo[0] + 1, ey = bx.hi[1] - bx.lo[1] + 1, ez = bx.hi[2] - bx.lo[2] + 1;
+    if (ex <= 0 || ey <= 0 || ez <= 0) return MF_ERR_BAD_ARG;
+    const int64_t tot = (int64_t)ex * ey * ez * c->nv;
+    int grid = (int)std::min<int64_t>((tot + 255) / 256, 65535);
+    hipLaunchKernelGGL(k_combine, dim3(grid), dim3(256), 0, c->stream, stage, from_array4(s_new), from_array4(src),
+                       from_array4(dsdt), bx.lo[0], bx.lo[1], bx.lo[2], ex, ey, ez, c->nv, dt);
+    HIPCHK(hipGetLastError());
+    return MF_OK;
+}
+
+// ----- level forms -------------------------------------------------------------
+int mf_level_create(void *ctx, const mf_box *boxes, int nboxes, void **level_out) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c || !level_out || nboxes < 0 || (nboxes > 0 && !boxes)) return MF_ERR_BAD_ARG;
+    *level_out = nullptr;
+    Level *L = new (std::nothrow) Level();
+    if (!L) return MF_ERR_BAD_ARG;
+    L->c = c;
+    for (int d = 0; d < 3; ++d) L->n[d] = c->geom.n_cell[d];
+    const int64_t ncell = (int64_t)L->n[0] * L->n[1] * L->n[2];
+    if (nboxes == 0) {
+        mf_box b;
+        for (int d = 0; d < 3; ++d) { b.lo[d] = 0; b.hi[d] = L->n[d] - 1; }
+        L->boxes.push_back(b);
+    } else {
+        int64_t cov = 0;
+        for (int b = 0; b < nboxes; ++b) {
+            for (int d = 0; d < 3; ++d)
+                if (boxes[b].lo[d] < 0 || boxes[b].hi[d] >= L->n[d] || boxes[b].hi[d] < boxes[b].lo[d]) { delete L; return MF_ERR_BAD_ARG; }
+            cov += (int64_t)(boxes[b].hi[0] - boxes[b].lo[0] + 1) * (boxes[b].hi[1] - boxes[b].lo[1] + 1) *
+                   (boxes[b].hi[2] - boxes[b].lo[2] + 1);
+            L->boxes.push_back(boxes[b]);
+        }
+        if (cov != ncell) { delete L; return MF_ERR_BAD_ARG; }  // boxes must tile the domain
+    }
+    const int NC = NCV + c->ns;
+    L->padded = (size_t)(L->n[0] + 6) * (L->n[1] + 6) * (L->n[2] + 6) * c->nv;
+    size_t fl = 0;
+    for (const mf_box &B : L->boxes) {
+        size_t e[3];
+        for (int d = 0; d < 3; ++d) e[d] = B.hi[d] - B.lo[d] + 1;
+        fl += ((e[0] + 1) * e[1] * e[2] + e[0] * (e[1] + 1) * e[2] + e[0] * e[1] * (e[2] + 1)) * NC;
+    }
+    L->flux_elems = fl;
+    bool ok = true;
+    for (int q = 0; q < 3 && ok; ++q) ok = hipMalloc(&L->arr[q], L->padded * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&L->flux, fl * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&L->cutid, sizeof(int) * ncell) == hipSuccess;
+    ok = ok && hipMalloc(&L->ts, sizeof(TimeState)) == hipSuccess;
+    if (!ok) {
+        mf_level_destroy(L);
+        return MF_ERR_HIP;
+    }
+    for (int q = 0; q < 3; ++q) {
+        hipMemset(L->arr[q], 0, L->padded * sizeof(double));
+        L->view[q] = padded_view(L->arr[q], L->n, c->nv);
+    }
+    hipMemset(L->cutid, 0xff, sizeof(int) * ncell);
+    TimeState h;
+    std::memset(&h, 0, sizeof(h));
+    h.t = 0.0;
+    h.dt = 0.0;
+    h.dt_prev = 1.e100;
+    h.stop_time = c->prm.stop_time;
+    h.active = 1;
+    hipMemcpy(L->ts, &h, sizeof(h), hipMemcpyHostToDevice);
+    int st = rebuild_tiles(L, c->tsh);
+    if (st) { mf_level_destroy(L); return st; }
+    *level_out = L;
+    return MF_OK;
+}
+
+int mf_level_destroy(void *level) {
+    Level *L = (Level *)level;
+    if (!L) return MF_ERR_BAD_ARG;
+    if (L->c) hipStreamSynchronize(L->c->stream);
+    if (L->gexec) hipGraphExecDestroy(L->gexec);
+    if (L->graph) hipGraphDestroy(L->graph);
+    for (int q = 0; q < 3; ++q) if (L->arr[q]) hipFree(L->arr[q]);
+    if (L->flux) hipFree(L->flux);
+    if (L->cutid) hipFree(L->cutid);
+    if (L->cutlist) hipFree(L->cutlist);
+    if (L->cutbuf) hipFree(L->cutbuf);
+    if (L->partials) hipFree(L->partials);
+    if (L->d_box) hipFree(L->d_box);
+    if (L->ts) hipFree(L->ts);
+    delete L;
+    return MF_OK;
+}
+
+int mf_level_set_state(void *level, const double *state, int is_device) {
+    Level *L = (Level *)level;
+    if (!L || !state) return MF_ERR_BAD_ARG;
+    Ctx *c = L->c;
+    const int64_t ncell = (int64_t)L->n[0] * L->n[1] * L->n[2];
+    const size_t bytes = (size_t)ncell * c->nv * sizeof(double);
+    const double *dsrc = state;
+    double *tmp = nullptr;
+    if (!is_device) {
+        HIPCHK(hipMalloc(&tmp, bytes));
+        HIPCHK(hipMemcpy(tmp, state, bytes, hipMemcpyHostToDevice));
+        dsrc = tmp;
+    }
+    hipLaunchKernelGGL(k_pack, dim3(4096), dim3(256), 0, c->stream, dsrc, L->view[0], L->n[0], L->n[1], L->n[2], c->nv, 1);
+    std::vector<double> vf(ncell);
+    if (!is_device) std::memcpy(vf.data(), state + (size_t)VFRAC * ncell, ncell * sizeof(double));
+    else HIPCHK(hipMemcpy(vf.data(), state + (size_t)VFRAC * ncell, ncell * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (tmp) hipFree(tmp);
+    int st = build_cutlist(L, vf.data());
+    if (st) return st;
+    L->need_est = true;
+    return MF_OK;
+}
+
+int mf_level_get_state(void *level, double *state, int is_device) {
+    Level *L = (Level *)level;
+    if (!L || !state) return MF_ERR_BAD_ARG;
+    Ctx *c = L->c;
+    const int64_t ncell = (int64_t)L->n[0] * L->n[1] * L->n[2];
+    const size_t bytes = (size_t)ncell * c->nv * sizeof(double);
+    double *dst = state;
+    double *tmp = nullptr;
+    if (!is_device) {
+        HIPCHK(hipMalloc(&tmp, bytes));
+        dst = tmp;
+    }
+    hipLaunchKernelGGL(k_pack, dim3(4096), dim3(256), 0, c->stream, dst, L->view[0], L->n[0], L->n[1], L->n[2], c->nv, 0);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (tmp) {
+        HIPCHK(hipMemcpy(state, tmp, bytes, hipMemcpyDeviceToHost));
+        hipFree(tmp);
+    }
+    return MF_OK;
+}
+
+int mf_level_state_view(void *level, mf_array4 *v) {
+    Level *L = (Level *)level;
+    if (!L || !v) return MF_ERR_BAD_ARG;
+    View w = L->view[0];
+    v->p = w.p;
+    v->jstride = w.js;
+    v->kstride = w.ks;
+    v->nstride = w.ns;
+    for (int d = 0; d < 3; ++d) {
+        v->begin[d] = -MF_NGROW;
+        v->end[d] = L->n[d] + MF_NGROW;
+    }
+    v->ncomp = L->c->nv;
+    v->pad_ = 0;
+    return MF_OK;
+}
+
+int mf_fill_ghosts_single_level(void *level, double time) {
+    Level *L = (Level *)level;
+    if (!L) return MF_ERR_BAD_ARG;
+    Ctx *c = L->c;
+    FillArgs f = fill_args(L, L->arr[0], 1);
+    f.ts = nullptr;
+    f.time_host = time;
+    for (int ph = 0; ph <= 3; ++ph) c->ks->fill(f, ph, c->stream);
+    HIPCHK(hipGetLastError());
+    return MF_OK;
+}
+
+int mf_est_dt(void *level, double *dt_out) {
+    Level *L = (Level *)level;
+    if (!L || !dt_out) return MF_ERR_BAD_ARG;
+    Ctx *c = L->c;
+    enqueue_est(L);
+    HIPCHK(hipGetLastError());
+    TimeState h;
+    int st = read_ts(L, &h);
+    if (st) return st;
+    L->need_est = false;
+    double est = DBL_MAX;
+    if (h.max_ws > 0)
+        for (int d = 0; d < 3; ++d) est = std::min(est, c->dx[d] / h.max_ws);
+    if (h.max_D > 0)
+        for (int d = 0; d < 3; ++d) est = std::min(est, 0.5 * c->dx[d] * c->dx[d] / h.max_D / 3);
+    *dt_out = est * c->prm.cfl;
+    return MF_OK;
+}
+
+int mf_level_step(void *level, int nsteps, int mode) {
+    Level *L = (Level *)level;
+    if (!L || nsteps < 0 || mode < 0 || mode > 2) return MF_ERR_BAD_ARG;
+    Ctx *c = L->c;
+    if (L->need_est) {
+        enqueue_est(L);
+        L->need_est = false;
+    }
+    if (mode == MF_MODE_GRAPH) {
+        if (!L->gexec) {
+            hipStream_t cap = c->stream;
+            bool own = false;
+            if (!cap) {
+                HIPCHK(hipStreamCreate(&cap));
+                own = true;
+            }
+            HIPCHK(hipStreamSynchronize(c->stream));
+            hipStream_t saved = c->stream;
+            c->stream = cap;
+            HIPCHK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+            enqueue_step(L, MF_MODE_BATCHED);
+            hipError_t e = hipStreamEndCapture(cap, &L->graph);
+            c->stream = saved;
+            if (own) hipStreamDestroy(cap);
+            if (e != hipSuccess) { last_hip_error = e; return MF_ERR_HIP; }
+            HIPCHK(hipGraphInstantiate(&L->gexec, L->graph, nullptr, nullptr, 0));
+        }
+        for (int s = 0; s < nsteps; ++s) HIPCHK(hipGraphLaunch(L->gexec, c->stream));
+    } else {
+        for (int s = 0; s < nsteps; ++s) enqueue_step(L, mode);
+    }
+    HIPCHK(hipGetLastError());
+    TimeState h;
+    int st = read_ts(L, &h);
+    if (st) return st;
+    return h.err;
+}
+
+int mf_level_info(void *level, double *time, double *dt, int *istep, double *resnorms) {
+    Level *L = (Level *)level;
+    if (!L) return MF_ERR_BAD_ARG;
+    TimeState h;
+    int st = read_ts(L, &h);
+    if (st) return st;
+    if (time) *time = h.t;
+    if (dt) *dt = h.dt;
+    if (istep) *istep = h.istep;
+    if (resnorms) for (int q = 0; q < L->c->nv; ++q) resnorms[q] = h.resnorm[q];
+    return MF_OK;
+}
+
+int mf_level_set_time(void *level, double time, double dt_prev, int istep) {
+    Level *L = (Level *)level;
+    if (!L) return MF_ERR_BAD_ARG;
+    TimeState h;
+    int st = read_ts(L, &h);
+    if (st) return st;
+    h.t = time;
+    h.dt_prev = dt_prev;
+    h.istep = istep;
+    h.active = 1;
+    h.err = 0;
+    HIPCHK(hipMemcpy(L->ts, &h, sizeof(h), hipMemcpyHostToDevice));
+    return MF_OK;
+}
+
+int mf_level_set_tile(void *level, int tx, int ty, int tz) {
+    Level *L = (Level *)level;
+    if (!L) return MF_ERR_BAD_ARG;
+    if (tx == 0 && ty == 0 && tz == 0) return mf_level_autotune(level);
+    if (tx < 1 || ty < 1 || tz < 1 || (int64_t)tx * ty * tz > 4096) return MF_ERR_BAD_ARG;
+    HIPCHK(hipStreamSynchronize(L->c->stream));
+    int t[3] = {tx, ty, tz};
+    return rebuild_tiles(L, t);
+}
+
+int mf_level_get_tile(void *level, int *tx, int *ty, int *tz) {
+    Level *L = (Level *)level;
+    if (!L) return MF_ERR_BAD_ARG;
+    if (tx) *tx = L->tsh[0];
+    if (ty) *ty = L->tsh[1];
+    if (tz) *tz = L->tsh[2];
+    return MF_OK;
+}
+
+// Autotuner: times stage 1 (ghost fill, cut cells, three face sweeps, residual
+// and combine into B) for each candidate tile shape; the state in A is only read.
+int mf_level_autotune(void *level) {
+    Level *L = (Level *)level;
+    if (!L) return MF_ERR_BAD_ARG;
+    Ctx *c = L->c;
+    static const int cand[][3] = {{32, 4, 2}, {64, 4, 1}, {16, 8, 2}, {32, 8, 1}, {64, 2, 2}, {16, 4, 4},
+                                  {128, 2, 1}, {8, 8, 4}, {16, 16, 1}, {32, 2, 4}};
+    const int ncand = sizeof(cand) / sizeof(cand[0]);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    TimeState saved;
+    int st = read_ts(L, &saved);
+    if (st) return st;
+    TimeState probe = saved;
+    probe.active = 1;
+    probe.err = 0;
+    if (!(probe.dt > 0)) probe.dt = 1e-30;
+    HIPCHK(hipMemcpy(L->ts, &probe, sizeof(probe), hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double best = 1e300;
+    int bi = 0;
+    for (int q = 0; q < ncand; ++q) {
+        bool fits = true;
+        for (int d = 0; d < 3; ++d) if (cand[q][d] > 2 * L->n[d] && cand[q][d] > 1) fits = false;
+        if (!fits) continue;
+        st = rebuild_tiles(L, cand[q]);
+        if (st) break;
+        enqueue_stage(L, 1, MF_MODE_BATCHED);  // warm-up
+        HIPCHK(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < 3; ++r) enqueue_stage(L, 1, MF_MODE_BATCHED);
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; bi = q; }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    HIPCHK(hipMemcpy(L->ts, &saved, sizeof(saved), hipMemcpyHostToDevice));
+    int ok = 0;
+    HIPCHK(hipMemcpy(c->d_err, &ok, sizeof(int), hipMemcpyHostToDevice));
+    if (st) return st;
+    return rebuild_tiles(L, cand[bi]);
+}
+
+int mf_util_copy(void *ctx, double *dst, const double *src, int64_t n) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c || (n > 0 && (!dst || !src))) return MF_ERR_BAD_ARG;
+    if (n <= 0) return MF_OK;
+    int grid = (int)std::min<int64_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, c->stream, dst, src, n);
+    HIPCHK(hipGetLastError());
+    return MF_OK;
+}
+
+int mf_adjoint_forward(void *level, int nsteps, int ncheck, int spec, double *J_out) {
+    (void)level; (void)nsteps; (void)ncheck; (void)spec; (void)J_out;
+    return MF_ERR_UNSUPPORTED;
+}
+int mf_adjoint_backward(void *level, double *grad_logA, double *grad_invel) {
+    (void)level; (void)grad_logA; (void)grad_invel;
+    return MF_ERR_UNSUPPORTED;
+}
+
+}  // extern "C"

@@ -1,0 +1,491 @@
+// mf_device.h -- device numerics of the RHS path for gfx950 (FP64 throughout).
+//
+// Each function states the reference lines whose arithmetic it reproduces.
+// Operation order follows the reference expression by expression (the build
+// turns FP contraction off), so that results agree with the reference's CPU
+// build to rounding. pow(x,2) is written x*x and pow(x,-1) as 1/x.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "mesoflow_amd.h"
+
+#define MFD __device__ __forceinline__
+
+namespace mfd {
+
+// Source/mflo_constants.H:4-28 (third/sixth are the products the macros expand to)
+constexpr double kZero = 0.0, kOne = 1.0, kTwo = 2.0, kThree = 3.0, kFour = 4.0, kFive = 5.0;
+constexpr double kTwo3rd = 0.6666666666666666667, kHalf = 0.5;
+constexpr double kThird = kTwo3rd * kHalf;
+constexpr double kEighth = 0.125, kSixteenth = 0.0625, kTwelveth = 0.08333333333;
+constexpr double kFourth = 0.25, kThreeTenth = 0.3, kThreeFourth = 0.75;
+constexpr double kTiny = 1e-20, kTiny2 = 1e-40;
+
+// Source/globalDefines.H:5-32
+enum : int { RHO = 0, RHOU = 1, RHOV = 2, RHOW = 3, RHOE = 4, DENS = 5, VELX = 6, VELY = 7,
+             VELZ = 8, PRES = 9, TEMP = 10, VFRAC = 11, FLO = 12, NCV = 5 };
+
+// ---------------------------------------------------------------------------
+// views
+// ---------------------------------------------------------------------------
+struct View {  // amrex::Array4 on the device
+    double *p;
+    int64_t js, ks, ns;
+    int b0, b1, b2;
+    MFD double &operator()(int i, int j, int k, int n) const {
+        return p[(int64_t)(i - b0) + (int64_t)(j - b1) * js + (int64_t)(k - b2) * ks + (int64_t)n * ns];
+    }
+    MFD int64_t off(int i, int j, int k) const {
+        return (int64_t)(i - b0) + (int64_t)(j - b1) * js + (int64_t)(k - b2) * ks;
+    }
+};
+
+// State accessor of one box's Sborder: the stage source array plus, for cells of
+// the box's own valid region that are cut cells, the values update_cutcells
+// produced (kept apart so that other boxes and the RK combine still see the
+// raw state; Source/mflo_evolve.cpp:509-517 updates only the valid tilebox).
+struct Acc {
+    View s;
+    const int *cutid;      // level valid cells (nx*ny*nz), -1 if not a cut cell; null: no substitution
+    const double *cutbuf;  // [nv][ncut]
+    int ncut;
+    int nx, ny;
+    int blo[3], bhi[3];    // owning box
+    MFD int cid(int i, int j, int k) const {
+        if (!cutid) return -1;
+        if (i < blo[0] || i > bhi[0] || j < blo[1] || j > bhi[1] || k < blo[2] || k > bhi[2]) return -1;
+        return cutid[(int64_t)i + (int64_t)nx * ((int64_t)j + (int64_t)ny * k)];
+    }
+    MFD double get(int i, int j, int k, int n, int c) const {
+        return c >= 0 ? cutbuf[(int64_t)n * ncut + c] : s(i, j, k, n);
+    }
+    MFD double operator()(int i, int j, int k, int n) const { return get(i, j, k, n, cid(i, j, k)); }
+};
+
+// One cell seen by a pointwise plugin (the reference's plugins read phi(i,j,k,*) only)
+struct CellRef {
+    const Acc *a;
+    int i, j, k, c;
+    MFD double operator()(int n) const { return a->get(i, j, k, n, c); }
+};
+// A cell held in registers
+template <int NV>
+struct CellArr {
+    const double *v;
+    MFD double operator()(int n) const { return v[n]; }
+};
+
+// ---------------------------------------------------------------------------
+// Plugins: compile-time families with runtime tables (mf_plugin_params)
+// ---------------------------------------------------------------------------
+using Tab = mf_plugin_params;
+
+// tests/Cartesian/ShockTube_N2/{thermo.H:19-133, transport.H, userfuncs.H, chemistry.H, externbc.H}
+struct PShockTubeN2 {
+    static constexpr int NS = 1;
+    static constexpr int NG = 1;
+    MFD static double get_t_from_rpc(const Tab &T, double rho, double pres, const double *spec) {
+        double sum_ci = kZero + spec[0];
+        return pres / (sum_ci * T.R_univ);
+    }
+    MFD static double get_p_from_rec(const Tab &T, double rhoe, double rho, const double *spec) {
+        double cv = T.R_univ / (T.gamma[0] - kOne);
+        double Tm = rhoe / (cv * spec[0]);
+        return kZero + spec[0] * T.R_univ * Tm;
+    }
+    MFD static double get_gama_from_rpc(const Tab &T, double rho, double pres, const double *spec) {
+        double sum_ci = kZero + spec[0];
+        double x0 = spec[0] / sum_ci;
+        return kOne + kOne / (x0 / (T.gamma[0] - kOne));
+    }
+    MFD static double get_e_from_rpc(const Tab &T, double rho, double pres, const double *spec) {
+        double cv = T.R_univ / (T.gamma[0] - kOne) / T.molwt[0];
+        double sum_ci = kZero + spec[0];
+        double x0 = spec[0] / sum_ci;
+        double temp = get_t_from_rpc(T, rho, pres, spec);
+        return temp * (cv * x0);
+    }
+    MFD static double get_solid_t_from_rhoe(const Tab &, double) { return 300.0; }
+    template <class C> MFD static void fluid_transport(const Tab &, const C &, double &visc, double &thcond) {
+        visc = kZero; thcond = kZero;
+    }
+    template <class C> MFD static void spec_dcoeff(const Tab &, const C &, double *d) { d[0] = kZero; }
+    template <class C> MFD static void fluid_source(const Tab &, const C &, double *src) {
+        for (int n = 0; n < FLO; ++n) src[n] = kZero;
+    }
+    static constexpr bool kHasFluidSource = false;
+    static constexpr bool kHasChemistry = false;
+    template <class C> MFD static void spec_source(const Tab &, const C &, double *src) { src[0] = kZero; }
+    MFD static void externalbc(const Tab &T, const double *x, const double *s_int, double *s_ext, int idir, int sgn,
+                               double time) {
+        s_ext[PRES] = s_int[PRES];
+        s_ext[DENS] = T.left_rho;
+        s_ext[TEMP] = T.left_T;
+        s_ext[VELX] = kZero; s_ext[VELY] = kZero; s_ext[VELZ] = kZero;
+        s_ext[VFRAC] = kOne;
+        s_ext[RHO] = s_ext[DENS];
+        s_ext[RHOU] = kZero; s_ext[RHOV] = kZero; s_ext[RHOW] = kZero;
+        s_ext[RHOE] = s_ext[DENS] * get_e_from_rpc(T, T.left_rho, T.left_p, T.left_spec);
+        s_ext[FLO] = T.left_spec[0];
+    }
+};
+
+// Mixture ideal gas over gas species 0..NG-1: models/catalystCylinder/thermo.H:36-156
+template <int NSv, int NGv>
+struct PMixture {
+    static constexpr int NS = NSv;
+    static constexpr int NG = NGv;
+    MFD static double get_r_from_c(const Tab &T, const double *spec) {
+        double rho = kZero;
+#pragma unroll
+        for (int s = 0; s < NG; ++s) rho += spec[s] * T.molwt[s];
+        return rho;
+    }
+    MFD static double get_t_from_rpc(const Tab &T, double rho, double pres, const double *spec) {
+        double sum_ci = kZero;
+#pragma unroll
+        for (int s = 0; s < NG; ++s) sum_ci += spec[s];
+        return pres / (sum_ci * T.R_univ);
+    }
+    MFD static double get_p_from_rec(const Tab &T, double rhoe, double rho, const double *spec) {
+        double cvs = kZero;
+#pragma unroll
+        for (int s = 0; s < NG; ++s) cvs += (T.R_univ / (T.gamma[s] - kOne)) * spec[s];
+        double Tm = rhoe / cvs;
+        double p = kZero;
+#pragma unroll
+        for (int s = 0; s < NG; ++s) p += spec[s] * T.R_univ * Tm;
+        return p;
+    }
+    MFD static double get_gama_from_rpc(const Tab &T, double rho, double pres, const double *spec) {
+        double sum_ci = kZero;
+#pragma unroll
+        for (int s = 0; s < NG; ++s) sum_ci += spec[s];
+        double term2 = kZero;
+#pragma unroll
+        for (int s = 0; s < NG; ++s) term2 += (spec[s] / sum_ci) / (T.gamma[s] - kOne);
+        return kOne + kOne / term2;
+    }
+    MFD static double get_e_from_rpc(const Tab &T, double rho, double pres, const double *spec) {
+        double temp = get_t_from_rpc(T, rho, pres, spec);
+        double cvc = kZero;
+#pragma unroll
+        for (int s = 0; s < NG; ++s) cvc += (T.R_univ / (T.gamma[s] - kOne)) * spec[s];
+        return cvc * temp / rho;
+    }
+    MFD static double get_solid_t_from_rhoe(const Tab &, double) { return 300.0; }
+    // models/catalystCylinder/userfuncs.H:27-37
+    template <class C> MFD static void fluid_transport(const Tab &T, const C &, double &visc, double &thcond) {
+        visc = T.visc; thcond = T.thcond;
+    }
+    // models/catalystCylinder/transport.H:12-27
+    template <class C> MFD static void spec_dcoeff(const Tab &T, const C &cell, double *d) {
+        double gasfrac = cell(VFRAC);
+        double solfrac = kOne - gasfrac;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) d[s] = T.dcoef_gas[s] * gasfrac + T.dcoef_solid[s] * solfrac;
+    }
+    template <class C> MFD static void fluid_source(const Tab &, const C &, double *src) {
+        for (int n = 0; n < FLO; ++n) src[n] = kZero;
+    }
+    static constexpr bool kHasFluidSource = false;
+};
+
+// models/catalystCylinder: chemistry.H:13-29, externbc.H:13-73
+struct PCatalystCylinder : PMixture<4, 3> {
+    static constexpr bool kHasChemistry = true;
+    template <class C> MFD static void spec_source(const Tab &T, const C &cell, double *src) {
+        for (int s = 0; s < NS; ++s) src[s] = kZero;
+        double reactrate = T.kconst * cell(FLO + 3) * cell(FLO + 1);
+        src[1] -= reactrate;
+        src[2] += reactrate;
+    }
+    MFD static void externalbc(const Tab &T, const double *x, const double *s_int, double *s_ext, int idir, int sgn,
+                               double time) {
+        double spec[NS] = {kZero, kZero, kZero, kZero};
+        if (idir == 0 && sgn > 0) {
+            s_ext[DENS] = T.in_rho;
+            s_ext[PRES] = s_int[PRES];
+            s_ext[VELX] = T.in_vel;
+            s_ext[VELY] = kZero;
+            s_ext[VELZ] = kZero;
+            spec[0] = (s_ext[DENS] - T.in_spec[1] * T.molwt[1]) / T.molwt[0];
+            spec[1] = T.in_spec[1];
+            s_ext[TEMP] = get_t_from_rpc(T, s_ext[DENS], s_ext[PRES], spec);
+        } else {
+            s_ext[DENS] = s_int[DENS];
+            s_ext[PRES] = T.out_pres;
+            s_ext[VELX] = s_int[VELX];
+            s_ext[VELY] = s_int[VELY];
+            s_ext[VELZ] = s_int[VELZ];
+            s_ext[TEMP] = s_int[TEMP];
+            for (int s = 0; s < NS; ++s) spec[s] = s_int[FLO + s];
+        }
+        s_ext[RHO] = s_ext[DENS];
+        s_ext[RHOU] = s_ext[DENS] * s_ext[VELX];
+        s_ext[RHOV] = s_ext[DENS] * s_ext[VELY];
+        s_ext[RHOW] = s_ext[DENS] * s_ext[VELZ];
+        s_ext[RHOE] = s_ext[DENS] * get_e_from_rpc(T, s_ext[DENS], s_ext[PRES], spec);
+        s_ext[RHOE] += kHalf * s_ext[DENS] * (s_ext[VELX] * s_ext[VELX] + s_ext[VELY] * s_ext[VELY] + s_ext[VELZ] * s_ext[VELZ]);
+        for (int s = 0; s < NS; ++s) s_ext[FLO + s] = spec[s];
+        s_ext[VFRAC] = s_int[VFRAC];
+    }
+};
+
+// Arrhenius mass-action plugin in the reference's plugin API (the loop of
+// py_utils/rxnscheme_writer/example/chemistry.H:14-55 with
+// k_r = exp(log_A_r - Ea_r/(R T))); inflow x-lo after
+// py_utils/rxnscheme_writer/example/externbc.H:16-52, other ext_dir faces as the
+// catalystCylinder outflow.
+template <int NSv>
+struct PPorous : PMixture<NSv, NSv - 1> {
+    using Base = PMixture<NSv, NSv - 1>;
+    static constexpr int NS = NSv;
+    static constexpr int NG = NSv - 1;
+    static constexpr bool kHasChemistry = true;
+    template <class C> MFD static void spec_source(const Tab &T, const C &cell, double *src) {
+        double wdot[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) wdot[s] = kZero;
+        double Tm = cell(TEMP);
+        for (int r = 0; r < T.nreact; ++r) {
+            double concmult = 1.0;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                double nu = T.stoich[r][s];
+                if (nu < 0.0) {
+                    double c = cell(FLO + s);
+                    double a = fabs(nu);
+                    concmult *= (a == 1.0) ? c : ((a == 2.0) ? c * c : pow(c, a));
+                }
+            }
+            double krate = exp(T.log_A[r] - T.Ea[r] / (T.R_univ * Tm));
+            double reactrate = krate * concmult;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                double nu = T.stoich[r][s];
+                if (fabs(nu) > 0.0) wdot[s] += nu * reactrate;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) src[s] = kZero + wdot[s];
+    }
+    MFD static void externalbc(const Tab &T, const double *x, const double *s_int, double *s_ext, int idir, int sgn,
+                               double time) {
+        double spec[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) spec[s] = kZero;
+        if (idir == 0 && sgn > 0) {
+            s_ext[PRES] = s_int[PRES];
+            s_ext[TEMP] = T.in_temp;
+            double ctot = s_ext[PRES] / (T.R_univ * s_ext[TEMP]);
+#pragma unroll
+            for (int s = 0; s < NG; ++s) spec[s] = T.in_spec[s] * ctot;
+            s_ext[DENS] = Base::get_r_from_c(T, spec);
+            s_ext[VELX] = T.in_vel;
+            s_ext[VELY] = kZero;
+            s_ext[VELZ] = kZero;
+            s_ext[VFRAC] = kOne;
+        } else {
+            s_ext[DENS] = s_int[DENS];
+            s_ext[PRES] = T.out_pres;
+            s_ext[VELX] = s_int[VELX];
+            s_ext[VELY] = s_int[VELY];
+            s_ext[VELZ] = s_int[VELZ];
+            s_ext[TEMP] = s_int[TEMP];
+            s_ext[VFRAC] = s_int[VFRAC];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) spec[s] = s_int[FLO + s];
+        }
+        s_ext[RHO] = s_ext[DENS];
+        s_ext[RHOU] = s_ext[DENS] * s_ext[VELX];
+        s_ext[RHOV] = s_ext[DENS] * s_ext[VELY];
+        s_ext[RHOW] = s_ext[DENS] * s_ext[VELZ];
+        s_ext[RHOE] = s_ext[DENS] * Base::get_e_from_rpc(T, s_ext[DENS], s_ext[PRES], spec);
+        s_ext[RHOE] += kHalf * s_ext[DENS] * (s_ext[VELX] * s_ext[VELX] + s_ext[VELY] * s_ext[VELY] + s_ext[VELZ] * s_ext[VELZ]);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) s_ext[FLO + s] = spec[s];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Numerics of Source/Src_3d
+// ---------------------------------------------------------------------------
+template <class P>
+struct Num {
+    static constexpr int NS = P::NS;
+    static constexpr int NC = NCV + NS;
+    static constexpr int NV = FLO + NS;
+
+    // consprim_funcs.H:9-23
+    MFD static void cons_to_prim(const Tab &T, const double *u, double *p) {
+        p[0] = u[RHO];
+        p[1] = u[RHOU] / u[RHO];
+        p[2] = u[RHOV] / u[RHO];
+        p[3] = u[RHOW] / u[RHO];
+        double rhoe = u[RHOE] - kHalf * u[RHO] * (p[1] * p[1] + p[2] * p[2] + p[3] * p[3]);
+        p[4] = P::get_p_from_rec(T, rhoe, u[RHO], u + NCV);
+    }
+    // consprim_funcs.H:26-44 (u holds the species on entry)
+    MFD static void prim_to_cons(const Tab &T, double *u, const double *p) {
+        u[RHO] = p[0];
+        u[RHOU] = p[0] * p[1];
+        u[RHOV] = p[0] * p[2];
+        u[RHOW] = p[0] * p[3];
+        u[RHOE] = u[RHO] * P::get_e_from_rpc(T, p[0], p[4], u + NCV);
+        u[RHOE] += kHalf * u[RHO] * (p[1] * p[1] + p[2] * p[2] + p[3] * p[3]);
+    }
+    // pressure and velocities used by geteulerflux (consprim_funcs.H:57-66)
+    MFD static void euler_prims(const Tab &T, const double *u, double &vx, double &vy, double &vz, double &p) {
+        double rho = u[RHO];
+        vx = u[RHOU] / rho;
+        vy = u[RHOV] / rho;
+        vz = u[RHOW] / rho;
+        double rhoe = u[RHOE] - kHalf * u[RHO] * (vx * vx + vy * vy + vz * vz);
+        p = P::get_p_from_rec(T, rhoe, u[RHO], u + NCV);
+    }
+    // component c of geteulerflux (consprim_funcs.H:68-80) for the unit normal of
+    // direction D; the products with zero normal components add exact zeros.
+    template <int D>
+    MFD static double euler_comp(int c, const double *u, double vx, double vy, double vz, double p) {
+        const double v[3] = {vx, vy, vz};
+        const double vn = v[D];
+        const double rho = u[RHO];
+        if (c == RHO) return rho * vn;
+        if (c >= RHOU && c <= RHOW) {
+            const int a = c - RHOU;
+            double m = rho * v[a];
+            return (a == D) ? (m * v[a] + p) : (m * vn);
+        }
+        if (c == RHOE) return (u[RHOE] + p) * vn;
+        return u[c] * vn;
+    }
+    // flux_schemes.H:10-46 (presR evaluated with the left species)
+    template <int D>
+    MFD static double max_eigenvalue(const Tab &T, const double *ul, const double *ur) {
+        double rhoL = ul[RHO], rhoR = ur[RHO];
+        double rhoeL = ul[RHOE] - kHalf * (ul[RHOU] * ul[RHOU] + ul[RHOV] * ul[RHOV] + ul[RHOW] * ul[RHOW]) / ul[RHO];
+        double rhoeR = ur[RHOE] - kHalf * (ur[RHOU] * ur[RHOU] + ur[RHOV] * ur[RHOV] + ur[RHOW] * ur[RHOW]) / ur[RHO];
+        double presL = P::get_p_from_rec(T, rhoeL, ul[RHO], ul + NCV);
+        double presR = P::get_p_from_rec(T, rhoeR, ur[RHO], ul + NCV);
+        double velnL = ul[RHOU + D] / ul[RHO];
+        double velnR = ur[RHOU + D] / ur[RHO];
+        double veln = fmax(fabs(velnL), fabs(velnR));
+        double gamaL = P::get_gama_from_rpc(T, rhoL, presL, ul + NCV);
+        double gamaR = P::get_gama_from_rpc(T, rhoR, presR, ur + NCV);
+        double ahalf = fmax(sqrt(gamaL * presL / rhoL), sqrt(gamaR * presR / rhoR));
+        return ahalf + veln;
+    }
+    // flux_schemes.H:105-141
+    MFD static double split_mach(double M, double beta, bool minus) {
+        if (minus) {
+            if (fabs(M) <= kOne) return -kFourth * (M - kOne) * (M - kOne) - beta * (M * M - kOne) * (M * M - kOne);
+            return kHalf * (M - fabs(M));
+        }
+        if (fabs(M) <= 1) return kFourth * (M + kOne) * (M + kOne) + beta * (M * M - kOne) * (M * M - kOne);
+        return kHalf * (M + fabs(M));
+    }
+    // flux_schemes.H:143-181
+    MFD static double split_pres(double M, double alpha, bool minus) {
+        if (minus) {
+            if (fabs(M) <= kOne) return kFourth * (M - kOne) * (M - kOne) * (kTwo + M) - alpha * M * (M * M - kOne) * (M * M - kOne);
+            return kHalf * (M - fabs(M)) / M;
+        }
+        if (fabs(M) <= kOne) return kFourth * (M + kOne) * (M + kOne) * (kTwo - M) + alpha * M * (M * M - kOne) * (M * M - kOne);
+        return kHalf * (M + fabs(M)) / M;
+    }
+    // flux_schemes.H:183-310, Minf = 0.3, normal = unit vector of D
+    template <int D>
+    MFD static void ausmp_up(const Tab &T, const double *ul, const double *ur, double *fh) {
+        const double Minf2 = kThreeTenth * kThreeTenth;
+        const double delta = kThreeTenth, beta = kEighth, sigma = kOne, Kp = kFourth, Ku = kThreeFourth;
+        double pl[5], pr[5];
+        cons_to_prim(T, ul, pl);
+        cons_to_prim(T, ur, pr);
+        double gamaL = P::get_gama_from_rpc(T, pl[0], pl[4], ul + NCV);
+        double gamaR = P::get_gama_from_rpc(T, pr[0], pr[4], ur + NCV);
+        // pl[1]*nx + pl[2]*ny + pl[3]*nz with one unit component
+        double vL = pl[1 + D], vR = pr[1 + D];
+        double HL = gamaL / (gamaL - kOne) * (pl[4] / pl[0]) + kHalf * (pl[1] * pl[1] + pl[2] * pl[2] + pl[3] * pl[3]);
+        double HR = gamaR / (gamaR - kOne) * (pr[4] / pr[0]) + kHalf * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
+        double rho_half = kHalf * (pl[0] + pr[0]);
+        double astar2 = kTwo * (gamaL - kOne) / (gamaL + kOne) * HL;
+        double ahatL = astar2 / fmax(sqrt(astar2), fabs(vL));
+        astar2 = kTwo * (gamaR - kOne) / (gamaR + kOne) * HR;
+        double ahatR = astar2 / fmax(sqrt(astar2), fabs(vR));
+        double a_half = fmin(ahatL, ahatR);
+        double ML = vL / a_half, MR = vR / a_half;
+        double MLplus = split_mach(ML, beta, false);
+        double MRminus = split_mach(MR, beta, true);
+        double M_half = MLplus + MRminus;
+        double M_bar2 = kHalf * (vL * vL + vR * vR) / (a_half * a_half);
+        double Mo2 = fmin(kOne, fmax(M_bar2, Minf2));
+        double Mo = sqrt(Mo2);
+        double fa_Mo = Mo * (kTwo - Mo);
+        M_half = M_half - (Kp / fa_Mo) * fmax(kOne - sigma * M_bar2, kZero) * (pr[4] - pl[4]) / (rho_half * a_half * a_half);
+        double mod_M_half = fabs(M_half);
+        if (mod_M_half <= delta) mod_M_half = (M_half * M_half + delta * delta) / (kTwo * delta);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            double fL = (c == RHOE) ? ul[RHOE] + pl[4] : ul[c];
+            double fR = (c == RHOE) ? ur[RHOE] + pr[4] : ur[c];
+            fh[c] = (kHalf * M_half * (fL + fR) - kHalf * mod_M_half * (fR - fL)) * a_half;
+        }
+        double alpha = kThree * kSixteenth * (-kFour + kFive * fa_Mo * fa_Mo);
+        double PLplus = split_pres(ML, alpha, false);
+        double PRminus = split_pres(MR, alpha, true);
+        double p_half = (PLplus * pl[4] + PRminus * pr[4]);
+        p_half = p_half - Ku * PLplus * PRminus * (pl[0] + pr[0]) * (fa_Mo * a_half) * (vR - vL);
+        // fhalf[RHOU+a] += p_half*n[a]: only the normal component changes
+        // (adding p_half*0 to the others is exact)
+        fh[RHOU + D] += p_half;
+    }
+    // WENO5-JS half of flux_schemes.H:400-421 (left-biased value at the face
+    // between s2 and s3) and :423-444 (right-biased value), eps = 1e-6
+    MFD static double weno_plus(double s0, double s1, double s2, double s3, double s4) {
+        const double eps = 1e-6;
+        double t, b2, b1, b0;
+        t = s0 - 2.0 * s1 + s2;
+        double q = s0 - 4.0 * s1 + 3.0 * s2;
+        b2 = (13.0 / 12.0) * (t * t) + 0.25 * (q * q);
+        t = s1 - 2.0 * s2 + s3;
+        b1 = (13.0 / 12.0) * (t * t) + 0.25 * (s1 - s3) * (s1 - s3);
+        t = s2 - 2.0 * s3 + s4;
+        q = 3.0 * s2 - 4.0 * s3 + s4;
+        b0 = (13.0 / 12.0) * (t * t) + 0.25 * (q * q);
+        b2 = 1.0 / ((eps + b2) * (eps + b2));
+        b1 = 1.0 / ((eps + b1) * (eps + b1));
+        b0 = 1.0 / ((eps + b0) * (eps + b0));
+        double a2 = b2, a1 = 6.0 * b1, a0 = 3.0 * b0;
+        double alpha1 = 1.0 / (a2 + a1 + a0);
+        double v2 = 2.0 * s0 - 7.0 * s1 + 11.0 * s2;
+        double v1 = -s1 + 5.0 * s2 + 2.0 * s3;
+        double v0 = 2.0 * s2 + 5.0 * s3 - s4;
+        return (1.0 / 6.0) * alpha1 * (a2 * v2 + a1 * v1 + a0 * v0);
+    }
+    MFD static double weno_minus(double s1, double s2, double s3, double s4, double s5) {
+        const double eps = 1e-6;
+        double t, q, b2, b1, b0;
+        t = s5 - 2.0 * s4 + s3;
+        q = s5 - 4.0 * s4 + 3.0 * s3;
+        b2 = (13.0 / 12.0) * (t * t) + 0.25 * (q * q);
+        t = s4 - 2.0 * s3 + s2;
+        b1 = (13.0 / 12.0) * (t * t) + 0.25 * (s4 - s2) * (s4 - s2);
+        t = s3 - 2.0 * s2 + s1;
+        q = 3.0 * s3 - 4.0 * s2 + s1;
+        b0 = (13.0 / 12.0) * (t * t) + 0.25 * (q * q);
+        b2 = 1.0 / ((eps + b2) * (eps + b2));
+        b1 = 1.0 / ((eps + b1) * (eps + b1));
+        b0 = 1.0 / ((eps + b0) * (eps + b0));
+        double a2 = b2, a1 = 6.0 * b1, a0 = 3.0 * b0;
+        double alpha1 = 1.0 / (a2 + a1 + a0);
+        double v2 = 11.0 * s3 - 7.0 * s4 + 2.0 * s5;
+        double v1 = -s4 + 5.0 * s3 + 2.0 * s2;
+        double v0 = 2.0 * s3 + 5.0 * s2 - s1;
+        return (1.0 / 6.0) * alpha1 * (a2 * v2 + a1 * v1 + a0 * v0);
+    }
+    // compute_flux_3d.H:16-22
+    MFD static double harmonic_mean(double d1, double d2) { return 2.0 * d1 * d2 / (d1 + d2 + 1e-30); }
+};
+
+}  // namespace mfd

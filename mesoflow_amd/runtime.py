@@ -1,0 +1,215 @@
+"""Python host side of the C-ABI (libmesoflow_amd.so): context and level objects.
+
+Mirrors the reference's per-level driver (mflo::EvolveAMR, Source/mflo_evolve.cpp:18-89)
+and its per-box advance (mflo::compute_dsdt_flow body, :579-652) on top of the
+C entry points declared in include/mesoflow_amd.h. There is no fallback: if the
+HIP library or a GPU is missing every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmesoflow_amd.so")
+_LIB = None
+
+# every function of include/mesoflow_amd.h with its argument types
+_P = C.POINTER
+SIGNATURES = {
+    "mf_status_string": ([C.c_int], C.c_char_p),
+    "mf_device_count": ([_P(C.c_int)], C.c_int),
+    "mf_ctx_create": ([_P(abi.PluginParams), _P(abi.Params), _P(abi.Geom), _P(C.c_void_p)], C.c_int),
+    "mf_ctx_destroy": ([C.c_void_p], C.c_int),
+    "mf_ctx_set_stream": ([C.c_void_p, C.c_void_p], C.c_int),
+    "mf_ctx_nvars": ([C.c_void_p, _P(C.c_int)], C.c_int),
+    "mf_ctx_sync": ([C.c_void_p], C.c_int),
+    "mf_box_cutcells": ([C.c_void_p, abi.Box, abi.Array4, C.c_double], C.c_int),
+    "mf_box_rhs": ([C.c_void_p, abi.Box, abi.Array4, abi.Array4, _P(abi.Array4), C.c_double], C.c_int),
+    "mf_box_rk3_combine": ([C.c_void_p, C.c_int, abi.Box, abi.Array4, abi.Array4, abi.Array4, C.c_double], C.c_int),
+    "mf_level_create": ([C.c_void_p, _P(abi.Box), C.c_int, _P(C.c_void_p)], C.c_int),
+    "mf_level_destroy": ([C.c_void_p], C.c_int),
+    "mf_level_set_state": ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),
+    "mf_level_get_state": ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),
+    "mf_level_state_view": ([C.c_void_p, _P(abi.Array4)], C.c_int),
+    "mf_fill_ghosts_single_level": ([C.c_void_p, C.c_double], C.c_int),
+    "mf_est_dt": ([C.c_void_p, _P(C.c_double)], C.c_int),
+    "mf_level_step": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "mf_level_info": ([C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_int), C.c_void_p], C.c_int),
+    "mf_level_set_time": ([C.c_void_p, C.c_double, C.c_double, C.c_int], C.c_int),
+    "mf_level_set_tile": ([C.c_void_p, C.c_int, C.c_int, C.c_int], C.c_int),
+    "mf_level_get_tile": ([C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int)], C.c_int),
+    "mf_level_autotune": ([C.c_void_p], C.c_int),
+    "mf_adjoint_forward": ([C.c_void_p, C.c_int, C.c_int, C.c_int, _P(C.c_double)], C.c_int),
+    "mf_adjoint_backward": ([C.c_void_p, C.c_void_p, _P(C.c_double)], C.c_int),
+    "mf_util_copy": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64], C.c_int),
+}
+
+
+def lib():
+    """Loads the HIP library (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libmesoflow_amd.so is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _LIB = L
+    return _LIB
+
+
+def check(st, where):
+    if st != abi.MF_OK:
+        raise abi.MFError(where, st)
+
+
+def device_count():
+    n = C.c_int(0)
+    lib().mf_device_count(C.byref(n))
+    return n.value
+
+
+class Context:
+    """mf_ctx: plugin tables, mflo.* parameters and geometry on the current HIP device."""
+
+    def __init__(self, case):
+        self.case = case
+        h = C.c_void_p()
+        check(lib().mf_ctx_create(C.byref(case.plugin), C.byref(case.params), C.byref(case.geom), C.byref(h)),
+              "mf_ctx_create")
+        self.h = h
+        nv = C.c_int()
+        lib().mf_ctx_nvars(self.h, C.byref(nv))
+        self.nv = nv.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        st = lib().mf_ctx_sync(self.h)
+        check(st, "device work")
+
+    def set_stream(self, stream_ptr):
+        check(lib().mf_ctx_set_stream(self.h, C.c_void_p(stream_ptr)), "mf_ctx_set_stream")
+
+
+class Level:
+    """One single-level AMR level (mf_level_*): boxes tiling the domain, state in HBM."""
+
+    def __init__(self, ctx, boxes=None):
+        self.ctx = ctx
+        case = ctx.case
+        if boxes is None:
+            arr, nb = case.box_structs()
+        else:
+            arr = (abi.Box * len(boxes))()
+            for i, (lo, hi) in enumerate(boxes):
+                arr[i] = abi.make_box(lo, hi)
+            nb = len(boxes)
+        h = C.c_void_p()
+        check(lib().mf_level_create(ctx.h, arr, nb, C.byref(h)), "mf_level_create")
+        self.h = h
+        self.nboxes = nb
+        self.shape = (ctx.nv,) + tuple(case.n_cell[::-1])
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mf_level_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, s):
+        if hasattr(s, "data_ptr"):          # a torch tensor on the device
+            assert tuple(s.shape) == self.shape and s.is_contiguous()
+            check(lib().mf_level_set_state(self.h, C.c_void_p(s.data_ptr()), 1), "mf_level_set_state")
+            return
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        assert s.shape == self.shape, (s.shape, self.shape)
+        check(lib().mf_level_set_state(self.h, s.ctypes.data_as(C.c_void_p), 0), "mf_level_set_state")
+
+    def get_state(self, out=None):
+        if out is not None and hasattr(out, "data_ptr"):
+            check(lib().mf_level_get_state(self.h, C.c_void_p(out.data_ptr()), 1), "mf_level_get_state")
+            return out
+        o = np.empty(self.shape)
+        check(lib().mf_level_get_state(self.h, o.ctypes.data_as(C.c_void_p), 0), "mf_level_get_state")
+        return o
+
+    def step(self, n=1, mode=abi.MODE_BATCHED):
+        check(lib().mf_level_step(self.h, int(n), int(mode)), "mf_level_step")
+
+    def est_dt(self):
+        d = C.c_double()
+        check(lib().mf_est_dt(self.h, C.byref(d)), "mf_est_dt")
+        return d.value
+
+    def fill_ghosts(self, time=0.0):
+        check(lib().mf_fill_ghosts_single_level(self.h, float(time)), "mf_fill_ghosts_single_level")
+
+    def padded_state(self):
+        """Copy of the padded level array (ghost layers included) as numpy."""
+        import torch
+        v = abi.Array4()
+        check(lib().mf_level_state_view(self.h, C.byref(v)), "mf_level_state_view")
+        self.ctx.sync()
+        n = [v.end[d] - v.begin[d] for d in range(3)]
+        count = v.nstride * v.ncomp
+        buf = np.empty(count)
+        import ctypes
+        hip = _hip_memcpy()
+        hip(buf.ctypes.data_as(C.c_void_p), C.c_void_p(v.p), C.c_size_t(count * 8), 2)
+        return buf.reshape(v.ncomp, n[2], n[1], n[0])
+
+    def info(self):
+        t, dt, ist = C.c_double(), C.c_double(), C.c_int()
+        res = np.zeros(self.ctx.nv)
+        check(lib().mf_level_info(self.h, C.byref(t), C.byref(dt), C.byref(ist), res.ctypes.data_as(C.c_void_p)),
+              "mf_level_info")
+        return t.value, dt.value, ist.value, res
+
+    def set_time(self, t, dt_prev, istep):
+        check(lib().mf_level_set_time(self.h, float(t), float(dt_prev), int(istep)), "mf_level_set_time")
+
+    def set_tile(self, tx, ty, tz):
+        check(lib().mf_level_set_tile(self.h, int(tx), int(ty), int(tz)), "mf_level_set_tile")
+
+    def get_tile(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        check(lib().mf_level_get_tile(self.h, C.byref(a), C.byref(b), C.byref(c)), "mf_level_get_tile")
+        return a.value, b.value, c.value
+
+    def autotune(self):
+        check(lib().mf_level_autotune(self.h), "mf_level_autotune")
+        return self.get_tile()
+
+
+_HIPMEMCPY = None
+
+
+def _hip_memcpy():
+    global _HIPMEMCPY
+    if _HIPMEMCPY is None:
+        h = C.CDLL("libamdhip64.so")
+        f = h.hipMemcpy
+        f.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        f.restype = C.c_int
+        _HIPMEMCPY = f
+    return _HIPMEMCPY

@@ -1,0 +1,164 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes loader of the CPU oracle (liboracle.so).
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+the checker; never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from mesoflow_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build(force=False):
+    path = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(path):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "all"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return path
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        P = C.POINTER
+        L.orc_level_create.argtypes = [P(abi.PluginParams), P(abi.Params), P(abi.Geom), P(abi.Box), C.c_int,
+                                       P(C.c_void_p)]
+        L.orc_level_create.restype = C.c_int
+        L.orc_level_destroy.argtypes = [C.c_void_p]
+        L.orc_level_set_state.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_level_get_state.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_level_step.argtypes = [C.c_void_p, C.c_int]
+        L.orc_level_step.restype = C.c_int
+        L.orc_level_est_dt.argtypes = [C.c_void_p, P(C.c_double)]
+        L.orc_level_est_dt.restype = C.c_int
+        L.orc_level_fill_ghosts.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
+        L.orc_level_fill_ghosts.restype = C.c_int
+        L.orc_level_info.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int), C.c_void_p]
+        L.orc_level_set_time.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int]
+        L.orc_box_rhs.argtypes = [P(abi.PluginParams), P(abi.Params), P(abi.Geom), abi.Box, abi.Array4,
+                                  abi.Array4, P(abi.Array4), C.c_double]
+        L.orc_box_rhs.restype = C.c_int
+        L.orc_box_cutcells.argtypes = [P(abi.PluginParams), P(abi.Params), P(abi.Geom), abi.Box, abi.Array4]
+        L.orc_box_cutcells.restype = C.c_int
+        L.orc_kat.argtypes = [P(abi.PluginParams), C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_kat.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class OracleLevel:
+    """One single-level AMR level of the oracle, mirroring mf_level_* of the C-ABI."""
+
+    def __init__(self, case):
+        self.case = case
+        boxes, nb = case.box_structs()
+        h = C.c_void_p()
+        st = lib().orc_level_create(C.byref(case.plugin), C.byref(case.params), C.byref(case.geom), boxes, nb,
+                                    C.byref(h))
+        if st != 0:
+            raise abi.MFError("orc_level_create", st)
+        self.h = h
+        self.nv = case.nv
+        self.shape = (self.nv,) + tuple(case.n_cell[::-1])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_level_destroy(self.h)
+            self.h = None
+
+    def set_state(self, s):
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        assert s.shape == self.shape
+        lib().orc_level_set_state(self.h, _ptr(s))
+
+    def get_state(self):
+        out = np.empty(self.shape)
+        lib().orc_level_get_state(self.h, _ptr(out))
+        return out
+
+    def step(self, n=1):
+        st = lib().orc_level_step(self.h, int(n))
+        if st != 0:
+            raise abi.MFError("orc_level_step", st)
+
+    def est_dt(self):
+        d = C.c_double()
+        lib().orc_level_est_dt(self.h, C.byref(d))
+        return d.value
+
+    def fill_ghosts(self, time=0.0):
+        nx, ny, nz = self.case.n_cell
+        out = np.zeros((self.nv, nz + 6, ny + 6, nx + 6))
+        lib().orc_level_fill_ghosts(self.h, _ptr(out), float(time))
+        return out
+
+    def info(self):
+        t, dt, ist = C.c_double(), C.c_double(), C.c_int()
+        res = np.zeros(self.nv)
+        lib().orc_level_info(self.h, C.byref(t), C.byref(dt), C.byref(ist), _ptr(res))
+        return t.value, dt.value, ist.value, res
+
+    def set_time(self, t, dt_prev, istep):
+        lib().orc_level_set_time(self.h, float(t), float(dt_prev), int(istep))
+
+
+def kat(plugin, func, inp, nout):
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    out = np.zeros(nout)
+    st = lib().orc_kat(C.byref(plugin), int(func), _ptr(inp), _ptr(out))
+    if st != 0:
+        raise abi.MFError("orc_kat", st)
+    return out
+
+
+def box_rhs(case, lo, hi, sborder, time=0.0, with_flux=False):
+    """Per-box RHS of the oracle on a (NV, nz+6, ny+6, nx+6) Sborder array of the box."""
+    nv = case.nv
+    sb = np.ascontiguousarray(sborder, dtype=np.float64)
+    glo = [lo[d] - 3 for d in range(3)]
+    ghi = [hi[d] + 4 for d in range(3)]
+    a_sb = abi.make_array4(sb.ctypes.data, glo, ghi, nv)
+    n = [hi[d] - lo[d] + 1 for d in range(3)]
+    dsdt = np.zeros((nv, n[2], n[1], n[0]))
+    a_ds = abi.make_array4(dsdt.ctypes.data, lo, [hi[d] + 1 for d in range(3)], nv)
+    fl = None
+    fluxes = []
+    if with_flux:
+        fl = (abi.Array4 * 3)()
+        for d in range(3):
+            fe = [hi[q] + 1 for q in range(3)]
+            fe[d] += 1
+            f = np.zeros((nv, fe[2] - lo[2], fe[1] - lo[1], fe[0] - lo[0]))
+            fluxes.append(f)
+            fl[d] = abi.make_array4(f.ctypes.data, lo, fe, nv)
+    st = lib().orc_box_rhs(C.byref(case.plugin), C.byref(case.params), C.byref(case.geom),
+                           abi.make_box(lo, hi), a_sb, a_ds, fl, float(time))
+    if st != 0:
+        raise abi.MFError("orc_box_rhs", st)
+    return (dsdt, fluxes) if with_flux else dsdt
+
+
+def box_cutcells(case, lo, hi, sborder):
+    sb = np.ascontiguousarray(sborder, dtype=np.float64).copy()
+    glo = [lo[d] - 3 for d in range(3)]
+    ghi = [hi[d] + 4 for d in range(3)]
+    a_sb = abi.make_array4(sb.ctypes.data, glo, ghi, case.nv)
+    st = lib().orc_box_cutcells(C.byref(case.plugin), C.byref(case.params), C.byref(case.geom),
+                                abi.make_box(lo, hi), a_sb)
+    if st != 0:
+        raise abi.MFError("orc_box_cutcells", st)
+    return sb
