@@ -1,0 +1,212 @@
+"""Benchmark of the MI355X RHS + RK3 path (one GPU).
+
+Workload (default): single-level 256^3 FP64 reacting flow through a porous solid:
+seeded packed spheres (radii U[6,14] cells, porosity 0.6) as the immersed boundary,
+5 species (4 gas + 1 site), 2 Arrhenius reactions, inflow u = 1 m/s at T = 700 K,
+WENO5 fluxes (order_hyp = 5), SSP-RK3, viscous/heat/species diffusion, explicit
+chemistry, CFL 0.5. A "step" is one RK3 step of the whole level with inputs
+resident in HBM.
+
+Prints one JSON line: cell-updates/s (value), the roofline of the dominant kernel
+measured with HIP events on the stream it runs on, and the CPU oracle timed on a
+bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "FP64 cell-updates/s per RK3 step, % HBM roofline; adjoint/forward cost ratio"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_case(name):
+    from mesoflow_amd import cases
+    if name == "porous256":
+        c = cases.porous_reactor(n_cell=(256, 256, 256), nspec=5, nreact=2, geometry="spheres", porosity=0.6,
+                                 rmin=6.0, rmax=14.0, order_hyp=5, cfl=0.3, seed=20261203)
+        desc = ("porous reactor 256^3 FP64, 5 species, 2 Arrhenius reactions, IB sphere pack r~U[6,14] "
+                "porosity 0.6, inflow u=1 m/s T=700 K, WENO5 (order_hyp=5), single level, SSP-RK3")
+    elif name == "sod64":
+        c = cases.shocktube_n2(n_cell=(64, 64, 64), prob_hi=(1.0, 1.0, 1.0), order_hyp=5, cfl=0.5)
+        desc = "Sod shock tube 64^3 FP64, 1 species, WENO5, SSP-RK3 (CPU-runnable reference case)"
+    elif name == "porous512":
+        c = cases.porous_reactor(n_cell=(512, 512, 512), nspec=10, nreact=8, geometry="grf", porosity=0.45,
+                                 corr_len=8.0, order_hyp=5, cfl=0.3, seed=20261207)
+        desc = "porous reactor 512^3 FP64, 10 species, 8 Arrhenius reactions, GRF IB porosity 0.45, WENO5, SSP-RK3"
+    else:
+        raise SystemExit("unknown config %s" % name)
+    return c, desc
+
+
+def nv_of(case):
+    return 12 + case.nspec
+
+
+def cpu_baseline(name, budget_s):
+    """Oracle (CPU restatement, OpenMP) on a bounded sample of the workload."""
+    from mesoflow_amd import cases
+    from oracle.pyoracle import OracleLevel
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if name == "sod64":
+        c = cases.shocktube_n2(n_cell=(64, 64, 64), prob_hi=(1.0, 1.0, 1.0), order_hyp=5, cfl=0.5)
+        sample = "Sod 64^3, WENO5"
+    else:
+        n = 64
+        c = cases.porous_reactor(n_cell=(n, n, n), nspec=5 if name != "porous512" else 10,
+                                 nreact=2 if name != "porous512" else 8,
+                                 geometry="spheres" if name != "porous512" else "grf", porosity=0.6 if name != "porous512" else 0.45,
+                                 rmin=6.0, rmax=14.0, corr_len=8.0, order_hyp=5, cfl=0.3, seed=20261203)
+        sample = "same physics on a 64^3 sub-problem (sphere pack, NS=%d)" % c.nspec
+    L = OracleLevel(c)
+    L.set_state(c.make_state())
+    cells = float(np.prod(c.n_cell))
+    t0 = time.time()
+    L.step(1)
+    t1 = time.time()
+    per = max(t1 - t0, 1e-3)
+    nmore = max(1, min(50, int(budget_s / per) - 1))
+    t2 = time.time()
+    L.step(nmore)
+    t3 = time.time()
+    rate = cells * nmore / (t3 - t2)
+    return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": "%s, %d RK3 steps after 1 warm-up step, %.1f s, OMP threads %d" % (sample, nmore, t3 - t2, threads)}
+
+
+def read_traffic(config):
+    path = os.path.join(ROOT, "profiles", "traffic_%s.txt" % config)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="porous256")
+    ap.add_argument("--mode", default="batched", choices=["per_box", "batched", "graph"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tile", default=None, help="tile shapes of the 4 kernel classes 'x,y,z;x,y,z;x,y,z;x,y,z' (skips the autotuner)")
+    ap.add_argument("--calibrate", action="store_true", help="plain copy kernel for PMC scale factors")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        raise SystemExit(2)
+
+    import torch
+    from mesoflow_amd import abi, runtime
+
+    if args.calibrate:
+        case, _ = make_case("sod64")
+        ctx = runtime.Context(case)
+        n = 1 << 28
+        a = torch.zeros(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            runtime.check(runtime.lib().mf_util_copy(ctx.h, b.data_ptr(), a.data_ptr(), n), "mf_util_copy")
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "mf_util_copy", "bytes_read": n * 8, "bytes_written": n * 8}))
+        return
+
+    case, desc = make_case(args.config)
+    nx, ny, nz = case.n_cell
+    cells = nx * ny * nz
+    nv = nv_of(case)
+    log("config %s: %s" % (args.config, desc))
+    t0 = time.time()
+    state = case.make_state()
+    log("initial state built in %.1f s (%.2f GB)" % (time.time() - t0, state.nbytes / 1e9))
+    dev_state = torch.from_numpy(state).to("cuda")
+    del state
+    ctx = runtime.Context(case)
+    lvl = runtime.Level(ctx)
+    lvl.set_state(dev_state)
+    del dev_state
+    torch.cuda.synchronize()
+    if args.tile:
+        lvl.set_tiles([[int(v) for v in part.split(",")] for part in args.tile.split(";")])
+    elif not args.no_autotune:
+        tile = lvl.autotune()
+        log("autotuned tile", tile)
+    mode = {"per_box": abi.MODE_PER_BOX, "batched": abi.MODE_BATCHED, "graph": abi.MODE_GRAPH}[args.mode]
+    if args.warmup:
+        lvl.step(args.warmup, mode)
+    torch.cuda.synchronize()
+
+    # timed region: K steps, synchronised on both sides; kernels timed with HIP
+    # events on the context stream (modes per_box / batched)
+    t0 = time.perf_counter()
+    if mode == abi.MODE_GRAPH:
+        lvl.step(args.steps, mode)
+        classes = None
+    else:
+        classes = lvl.step_timed(args.steps, mode)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    value = cells * args.steps / secs
+    info = lvl.info()
+    log("time %.6e dt %.3e step %d" % (info[0], info[1], info[2]))
+
+    roof = None
+    extra = {}
+    step_bytes = 64.0 * nv * cells  # minimal fused RK3 stream per cell-update
+    if classes:
+        log("kernel classes (ms, launches):", json.dumps(classes))
+        ms_faces, n_faces = classes["face_flux"]
+        # dominant kernel: the face-flux sweep. One launch covers one direction of
+        # one stage, i.e. 1/9 of a cell-update of the whole level.
+        avg_s = ms_faces / max(n_faces, 1) * 1e-3
+        alg_bytes = step_bytes / 9.0
+        achieved = alg_bytes / avg_s / 1e9
+        traffic = read_traffic(args.config)
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_faces (WENO5 face-flux sweep)",
+                "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes_per_launch": alg_bytes}
+        tot = sum(v[0] for v in classes.values())
+        extra["kernel_time_share"] = {k: round(v[0] / tot, 4) for k, v in classes.items()}
+    extra["step_hbm_frac"] = (step_bytes * args.steps / secs / 1e9) / HBM_PEAK_GBS
+    extra["tile"] = ";".join(",".join(str(v) for v in t) for t in lvl.tiles())
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("timing the CPU oracle on a sample ...")
+        cpu = cpu_baseline(args.config, args.cpu_budget)
+        log("cpu baseline", cpu)
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": secs / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded initial state of the named workload (no datasets)",
+        "config": {"workload": desc, "cells": cells, "nspec": case.nspec, "nv": nv, "order_hyp": int(case.params.order_hyp),
+                   "mode": args.mode, "boxes": lvl.nboxes},
+        "roofline": roof, "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+    lvl.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -213,15 +213,27 @@ int mf_est_dt(void *level, double *dt_out);
 enum { MF_MODE_PER_BOX = 0, MF_MODE_BATCHED = 1, MF_MODE_GRAPH = 2 };
 int mf_level_step(void *level, int nsteps, int mode);
 
+/* mf_level_step (modes MF_MODE_PER_BOX / MF_MODE_BATCHED) with a HIP event pair
+ * around every launch group on the context stream. class_ms[7] receives the
+ * summed event time of: 0 ComputeDt, 1 ghost fill, 2 cut cells, 3 face-flux
+ * sweeps, 4 update_residual + RK3 combine, 5 step close, 6 post-step pass
+ * (chemistry, primitives, clip, residual and dt partials); class_launches[7]
+ * (may be NULL) the number of groups. Instrumentation of this
+ * build; no counterpart in the reference. */
+int mf_level_step_timed(void *level, int nsteps, int mode, double *class_ms, int *class_launches);
+
 /* Time, last dt, step count and residual norms (TOTAL_NVARS doubles, may be NULL). */
 int mf_level_info(void *level, double *time, double *dt, int *istep, double *resnorms);
 /* Overrides the level time/dt history (restart). */
 int mf_level_set_time(void *level, double time, double dt_prev, int istep);
 
-/* Tile shape used by the level kernels (0 = autotune on the next step). */
-int mf_level_set_tile(void *level, int tx, int ty, int tz);
-int mf_level_get_tile(void *level, int *tx, int *ty, int *tz);
-/* Runs the autotuner now on the current state (state is restored afterwards). */
+/* Tile shape of a kernel class: 0, 1, 2 the LDS-staged WENO5 face sweeps of
+ * direction x, y, z; 3 the cell tiles (residual + RK3 combine, dt estimate,
+ * face sweeps of the other schemes). */
+int mf_level_set_tile(void *level, int cls, int tx, int ty, int tz);
+int mf_level_get_tile(void *level, int cls, int *tx, int *ty, int *tz);
+/* Autotuner: times every kernel class on the current state for a set of
+ * candidate tile shapes and keeps the fastest (the state is only read). */
 int mf_level_autotune(void *level);
 
 /* ------------------------------------------------------------------------ */

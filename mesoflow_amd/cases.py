@@ -394,7 +394,7 @@ def vfrac_from_corners(solid):
 
 
 def porous_reactor(n_cell=(256, 256, 256), nspec=5, nreact=2, geometry="spheres", porosity=0.6, seed=20261203,
-                   dx=4.0e-6, order_hyp=5, cfl=0.5, rmin=6.0, rmax=14.0, corr_len=8.0, max_grid_size=None,
+                   dx=4.0e-6, order_hyp=5, cfl=0.3, rmin=6.0, rmax=14.0, corr_len=8.0, max_grid_size=None,
                    boxes=None, in_vel=1.0, nsteps=50):
     t = porous_plugin(nspec=nspec, nreact=nreact, seed=seed, in_vel=in_vel)
     p = default_params(order_hyp=order_hyp, cfl=cfl, lo_bc=(3, 0, 0), hi_bc=(2, 0, 0))

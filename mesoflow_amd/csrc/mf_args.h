@@ -22,19 +22,25 @@ struct TimeState {
     double resnorm[kMaxNV];
 };
 
+// Tile classes: 0..2 the LDS-staged face sweeps of direction 0..2, 3 cells
+// (residual/combine, dt estimate and the face sweeps without LDS staging).
+constexpr int kTileClasses = 4;
+constexpr int kCellClass = 3;
+
 // One box of a level (or the single box of a per-box call).
 struct BoxDesc {
     int lo[3], hi[3];
-    int ntile[3];
-    int tile_start;
+    int ntile[kTileClasses][3];
+    int tile_start[kTileClasses];
     View flux[3];      // face-centred flux of each direction; begin = lo
 };
 
-// Tiles [t0, t1) of the box/tile list (boxes == nullptr: the single box `one`).
+// Tiles [t0, t1) of class cls of the box/tile list (boxes == nullptr: the single box `one`).
 struct Work {
     const BoxDesc *boxes;
     int nboxes;
     int t0, t1;
+    int cls;
     int tsh[3];
     BoxDesc one;
 };
@@ -103,8 +109,12 @@ struct KernelSet {
     void (*cut)(const CutArgs &, hipStream_t);
     void (*cut_box)(const CutArgs &, hipStream_t);
     void (*faces)(const StageArgs &, const Work &, int dir, int grid, int block, hipStream_t);
+    // LDS-staged WENO5 sweeps (order_hyp = 5); lds_bytes(tile) gives the shared memory of a tile
+    void (*faces_lds)(const StageArgs &, const Work &, int dir, int grid, int block, size_t lds, hipStream_t);
+    size_t (*lds_bytes)(const int *tsh, int dir);
     void (*residual)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
     void (*est)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
+    void (*post)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
 };
 
 extern const KernelSet ks_shocktube_n2;

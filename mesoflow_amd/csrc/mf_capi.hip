@@ -49,14 +49,34 @@ struct Ctx {
     int tsh[3] = {32, 4, 2};
 };
 
+// HIP-event timing of the kernel classes of a step (mf_level_step_timed)
+enum { KC_DT = 0, KC_FILL = 1, KC_CUT = 2, KC_FACES = 3, KC_RESID = 4, KC_FINAL = 5, KC_POST = 6, KC_N = 7 };
+struct Timer {
+    bool on = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+    void begin(int cls, hipStream_t st) {
+        if (!on) return;
+        hipEvent_t a, b;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        hipEventRecord(a, st);
+        ev.push_back({cls, {a, b}});
+    }
+    void end(hipStream_t st) {
+        if (!on) return;
+        hipEventRecord(ev.back().second.second, st);
+    }
+};
+
 struct Level {
     Ctx *c = nullptr;
+    Timer timer;
     int n[3];
     std::vector<mf_box> boxes;
     std::vector<BoxDesc> hbox;
     BoxDesc *d_box = nullptr;
-    int ntiles = 0;
-    int tsh[3] = {0, 0, 0};
+    int ntiles[kTileClasses] = {0, 0, 0, 0};
+    int tsh[kTileClasses][3] = {{64, 4, 1}, {32, 8, 1}, {32, 1, 8}, {64, 2, 2}};
     int max_grid = 2048;
     double *arr[3] = {nullptr, nullptr, nullptr};  // A, B, C
     size_t padded = 0;
@@ -182,23 +202,36 @@ __global__ void k_compute_dt(TimeState *ts, double dx0, double dx1, double dx2, 
 
 // reduces the per-tile partials; mode 0 closes a step, mode 1 only refreshes
 // the dt estimate inputs
-__global__ __launch_bounds__(256) void k_finalize(const double *partials, int ntiles, int nv, TimeState *ts, int *err,
+__global__ __launch_bounds__(1024) void k_finalize(const double *partials, int ntiles, int nv, TimeState *ts, int *err,
                                                   int mode, int track) {
-    __shared__ double red[256];
+    __shared__ double red[1024];
     const int np = nv + 2;
     if (mode == 0 && (!ts->active || ts->err)) {
         if (threadIdx.x == 0 && *err && !ts->err) ts->err = *err;
         return;
     }
-    double vals[kMaxNV + 2];
-    for (int q = 0; q < np; ++q) {
-        const bool is_max = q >= nv;
-        double v = 0.0;
-        for (int t = threadIdx.x; t < ntiles; t += blockDim.x) {
-            double x = partials[(size_t)t * np + q];
-            v = is_max ? fmax(v, x) : v + x;
+    // thread folds whole tiles (contiguous reads), then one block reduction
+    // per value; the arrays are indexed with constants only (no scratch)
+    double acc[kMaxNV + 2];
+#pragma unroll
+    for (int q = 0; q < kMaxNV + 2; ++q) acc[q] = 0.0;
+    for (int t = threadIdx.x; t < ntiles; t += blockDim.x) {
+        const double *pt = partials + (size_t)t * np;
+#pragma unroll
+        for (int q = 0; q < kMaxNV + 2; ++q) {
+            if (q < np) {
+                const double x = pt[q];
+                acc[q] = (q >= nv) ? fmax(acc[q], x) : acc[q] + x;
+            }
         }
-        red[threadIdx.x] = v;
+    }
+    double vals[kMaxNV + 2];
+#pragma unroll
+    for (int q = 0; q < kMaxNV + 2; ++q) {
+        vals[q] = 0.0;
+        if (q >= np) continue;
+        const bool is_max = q >= nv;
+        red[threadIdx.x] = acc[q];
         __syncthreads();
         for (int s = blockDim.x / 2; s > 0; s >>= 1) {
             if (threadIdx.x < s) {
@@ -210,13 +243,22 @@ __global__ __launch_bounds__(256) void k_finalize(const double *partials, int nt
         vals[q] = red[0];
         __syncthreads();
     }
+    double mws = 0.0, mD = 0.0;
+#pragma unroll
+    for (int q = 0; q < kMaxNV + 2; ++q) {
+        if (q == nv) mws = vals[q];
+        if (q == nv + 1) mD = vals[q];
+    }
     if (threadIdx.x != 0) return;
-    ts->max_ws = vals[nv];
-    ts->max_D = vals[nv + 1];
+    ts->max_ws = mws;
+    ts->max_D = mD;
     ts->have_est = 1;
     if (mode == 1) return;
-    if (track)
-        for (int q = 0; q < nv; ++q) ts->resnorm[q] = sqrt(vals[q]);
+    if (track) {
+#pragma unroll
+        for (int q = 0; q < kMaxNV; ++q)
+            if (q < nv) ts->resnorm[q] = sqrt(vals[q]);
+    }
     ts->t = ts->t + ts->dt;
     ts->dt_prev = ts->dt;
     ts->istep += 1;
@@ -256,16 +298,18 @@ StageArgs stage_args(Level *L, int stage) {
     return a;
 }
 
-Work level_work(Level *L) {
+Work level_work(Level *L, int cls) {
     Work w;
     w.boxes = L->d_box;
     w.nboxes = (int)L->boxes.size();
     w.t0 = 0;
-    w.t1 = L->ntiles;
-    for (int d = 0; d < 3; ++d) w.tsh[d] = L->tsh[d];
+    w.t1 = L->ntiles[cls];
+    w.cls = cls;
+    for (int d = 0; d < 3; ++d) w.tsh[d] = L->tsh[cls][d];
     std::memset(&w.one, 0, sizeof(w.one));
     return w;
 }
+int box_tiles(const BoxDesc &bd, int cls) { return bd.ntile[cls][0] * bd.ntile[cls][1] * bd.ntile[cls][2]; }
 
 FillArgs fill_args(Level *L, double *arr, int stage) {
     Ctx *c = L->c;
@@ -302,11 +346,12 @@ CutArgs cut_args(Level *L, double *arr) {
     return a;
 }
 
-int rebuild_tiles(Level *L, const int *tsh) {
+int rebuild_tiles(Level *L) {
     Ctx *c = L->c;
-    for (int d = 0; d < 3; ++d) L->tsh[d] = std::max(1, tsh[d]);
+    for (int q = 0; q < kTileClasses; ++q)
+        for (int d = 0; d < 3; ++d) L->tsh[q][d] = std::max(1, L->tsh[q][d]);
     const int NC = NCV + c->ns;
-    int tiles = 0;
+    int tiles[kTileClasses] = {0, 0, 0, 0};
     size_t fofs = 0;
     L->hbox.resize(L->boxes.size());
     for (size_t b = 0; b < L->boxes.size(); ++b) {
@@ -318,10 +363,12 @@ int rebuild_tiles(Level *L, const int *tsh) {
             bd.lo[d] = B.lo[d];
             bd.hi[d] = B.hi[d];
             len[d] = B.hi[d] - B.lo[d] + 1;
-            bd.ntile[d] = div_up(len[d], L->tsh[d]);
         }
-        bd.tile_start = tiles;
-        tiles += bd.ntile[0] * bd.ntile[1] * bd.ntile[2];
+        for (int q = 0; q < kTileClasses; ++q) {
+            for (int d = 0; d < 3; ++d) bd.ntile[q][d] = div_up(len[d], L->tsh[q][d]);
+            bd.tile_start[q] = tiles[q];
+            tiles[q] += box_tiles(bd, q);
+        }
         for (int d = 0; d < 3; ++d) {
             int fl[3] = {len[0], len[1], len[2]};
             fl[d] += 1;
@@ -336,11 +383,11 @@ int rebuild_tiles(Level *L, const int *tsh) {
             fofs += (size_t)v.ns * NC;
         }
     }
-    L->ntiles = tiles;
+    for (int q = 0; q < kTileClasses; ++q) L->ntiles[q] = tiles[q];
     if (L->d_box) hipFree(L->d_box);
     HIPCHK(hipMalloc(&L->d_box, sizeof(BoxDesc) * L->hbox.size()));
     HIPCHK(hipMemcpy(L->d_box, L->hbox.data(), sizeof(BoxDesc) * L->hbox.size(), hipMemcpyHostToDevice));
-    size_t need = (size_t)tiles * (c->nv + 2);
+    size_t need = (size_t)tiles[kCellClass] * (c->nv + 2);
     if (need > L->partials_cap) {
         if (L->partials) hipFree(L->partials);
         HIPCHK(hipMalloc(&L->partials, need * sizeof(double)));
@@ -386,32 +433,48 @@ int build_cutlist(Level *L, const double *vfrac_host) {
 
 // launch of the face and residual kernels, one launch per box (mode 0) or one
 // persistent launch over the whole tile list
+// persistent grid: at most max_grid blocks, a multiple of 8 (one share per XCD)
+int grid_for(int ntiles, int max_grid) {
+    int g = std::min(ntiles, max_grid);
+    if (g >= 8) g &= ~7;
+    return std::max(g, 1);
+}
+
+bool use_lds(Level *L) { return L->c->prm.order_hyp == 5 && L->c->prm.do_ns; }
+
 void launch_faces(Level *L, const StageArgs &a, int dir, int mode) {
     Ctx *c = L->c;
-    Work w = level_work(L);
+    const bool lds = use_lds(L);
+    const int cls = lds ? dir : kCellClass;
+    Work w = level_work(L, cls);
+    const size_t shm = lds ? c->ks->lds_bytes(L->tsh[cls], dir) : 0;
+    auto go = [&](int grid) {
+        if (lds) c->ks->faces_lds(a, w, dir, grid, 256, shm, c->stream);
+        else c->ks->faces(a, w, dir, grid, 256, c->stream);
+    };
     if (mode == MF_MODE_PER_BOX) {
         for (size_t b = 0; b < L->hbox.size(); ++b) {
             const BoxDesc &bd = L->hbox[b];
-            w.t0 = bd.tile_start;
-            w.t1 = bd.tile_start + bd.ntile[0] * bd.ntile[1] * bd.ntile[2];
-            c->ks->faces(a, w, dir, w.t1 - w.t0, 256, c->stream);
+            w.t0 = bd.tile_start[cls];
+            w.t1 = w.t0 + box_tiles(bd, cls);
+            go(w.t1 - w.t0);
         }
     } else {
-        c->ks->faces(a, w, dir, std::min(L->ntiles, L->max_grid), 256, c->stream);
+        go(grid_for(L->ntiles[cls], L->max_grid));
     }
 }
 void launch_residual(Level *L, const ResArgs &r, int mode) {
     Ctx *c = L->c;
-    Work w = level_work(L);
+    Work w = level_work(L, kCellClass);
     if (mode == MF_MODE_PER_BOX) {
         for (size_t b = 0; b < L->hbox.size(); ++b) {
             const BoxDesc &bd = L->hbox[b];
-            w.t0 = bd.tile_start;
-            w.t1 = bd.tile_start + bd.ntile[0] * bd.ntile[1] * bd.ntile[2];
+            w.t0 = bd.tile_start[kCellClass];
+            w.t1 = w.t0 + box_tiles(bd, kCellClass);
             c->ks->residual(r, w, w.t1 - w.t0, 256, c->stream);
         }
     } else {
-        c->ks->residual(r, w, std::min(L->ntiles, L->max_grid), 256, c->stream);
+        c->ks->residual(r, w, grid_for(L->ntiles[kCellClass], L->max_grid), 256, c->stream);
     }
 }
 
@@ -419,32 +482,67 @@ void enqueue_stage(Level *L, int stage, int mode) {
     Ctx *c = L->c;
     double *src = L->arr[stage == 1 ? 0 : (stage == 2 ? 1 : 2)];
     FillArgs f = fill_args(L, src, stage);
+    L->timer.begin(KC_FILL, c->stream);
     for (int ph = 0; ph <= 3; ++ph) c->ks->fill(f, ph, c->stream);
-    if (L->ncut) c->ks->cut(cut_args(L, src), c->stream);
+    L->timer.end(c->stream);
+    if (L->ncut) {
+        L->timer.begin(KC_CUT, c->stream);
+        c->ks->cut(cut_args(L, src), c->stream);
+        L->timer.end(c->stream);
+    }
     StageArgs a = stage_args(L, stage);
-    for (int d = 0; d < 3; ++d) launch_faces(L, a, d, mode);
+    for (int d = 0; d < 3; ++d) {
+        L->timer.begin(KC_FACES, c->stream);
+        launch_faces(L, a, d, mode);
+        L->timer.end(c->stream);
+    }
     ResArgs r;
     r.st = a;
     r.combine = 1;
     std::memset(&r.dsdt, 0, sizeof(r.dsdt));
     r.old = L->view[0];
-    r.dst = L->view[stage == 1 ? 1 : (stage == 2 ? 2 : 0)];
-    r.epilogue = (stage == 3);
+    // stage 3 writes A in place with chemistry; without, B keeps S3 so that the
+    // post pass still sees U^n in A for the residual
+    r.dst = L->view[stage == 1 ? 1 : (stage == 2 ? 2 : (c->prm.do_chemistry ? 0 : 1))];
+    r.epilogue = 0;
     r.do_chem = c->prm.do_chemistry;
     r.clip = c->prm.clip_species;
     r.track_res = c->prm.track_residual_norms;
     r.do_ns = c->prm.do_ns;
     r.partials = L->partials;
+    L->timer.begin(KC_RESID, c->stream);
     launch_residual(L, r, mode);
+    L->timer.end(c->stream);
+    if (stage == 3) {
+        r.st.src = r.dst;
+        r.dst = L->view[0];
+        L->timer.begin(KC_POST, c->stream);
+        Work w = level_work(L, kCellClass);
+        if (mode == MF_MODE_PER_BOX) {
+            for (size_t b = 0; b < L->hbox.size(); ++b) {
+                const BoxDesc &bd = L->hbox[b];
+                w.t0 = bd.tile_start[kCellClass];
+                w.t1 = w.t0 + box_tiles(bd, kCellClass);
+                c->ks->post(r, w, w.t1 - w.t0, 256, c->stream);
+            }
+        } else {
+            c->ks->post(r, w, grid_for(L->ntiles[kCellClass], L->max_grid), 256, c->stream);
+        }
+        L->timer.end(c->stream);
+    }
 }
 
 void enqueue_step(Level *L, int mode) {
     Ctx *c = L->c;
+    L->timer.begin(KC_DT, c->stream);
     hipLaunchKernelGGL(k_compute_dt, dim3(1), dim3(1), 0, c->stream, L->ts, c->dx[0], c->dx[1], c->dx[2], c->prm.cfl,
                        c->prm.fixed_dt, c->prm.dt_fixed);
+    L->timer.end(c->stream);
     for (int s = 1; s <= 3; ++s) enqueue_stage(L, s, mode);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, L->partials, L->ntiles, c->nv, L->ts, c->d_err, 0,
+    L->timer.begin(KC_FINAL, c->stream);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, L->partials, L->ntiles[kCellClass], c->nv, L->ts, c->d_err, 0,
                        c->prm.track_residual_norms);
+    L->timer.end(c->stream);
 }
 
 void enqueue_est(Level *L) {
@@ -455,9 +553,9 @@ void enqueue_est(Level *L) {
     r.st.ts = nullptr;
     r.do_ns = c->prm.do_ns;
     r.partials = L->partials;
-    Work w = level_work(L);
-    c->ks->est(r, w, std::min(L->ntiles, L->max_grid), 256, c->stream);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c->stream, L->partials, L->ntiles, c->nv, L->ts, c->d_err, 1, 0);
+    Work w = level_work(L, kCellClass);
+    c->ks->est(r, w, std::min(L->ntiles[kCellClass], L->max_grid), 256, c->stream);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, L->partials, L->ntiles[kCellClass], c->nv, L->ts, c->d_err, 1, 0);
 }
 
 int read_ts(Level *L, TimeState *h) {
@@ -531,8 +629,16 @@ int mf_ctx_create(const mf_plugin_params *plugin, const mf_params *params, const
     c->nv = FLO + c->ns;
     for (int d = 0; d < 3; ++d) c->dx[d] = (geom->prob_hi[d] - geom->prob_lo[d]) / geom->n_cell[d];
     hipGetDevice(&c->device);
+    Tab ht;
+    std::memset(&ht, 0, sizeof(ht));
+    std::memcpy(static_cast<mf_plugin_params *>(&ht), plugin, sizeof(mf_plugin_params));
+    for (int q = 0; q < MF_MAX_SPECIES; ++q) {
+        ht.gm1[q] = plugin->gamma[q] - 1.0;
+        ht.cv[q] = plugin->R_univ / (plugin->gamma[q] - 1.0);
+        ht.cvm[q] = plugin->R_univ / (plugin->gamma[q] - 1.0) / plugin->molwt[q];
+    }
     if (hipMalloc(&c->d_tab, sizeof(Tab)) != hipSuccess || hipMalloc(&c->d_err, sizeof(int)) != hipSuccess ||
-        hipMemcpy(c->d_tab, plugin, sizeof(Tab), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_tab, &ht, sizeof(Tab), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->d_err, 0, sizeof(int)) != hipSuccess) {
         delete c;
         return MF_ERR_HIP;
@@ -617,14 +723,15 @@ int mf_box_rhs(void *ctx, mf_box bx, mf_array4 sborder, mf_array4 dsdt, const mf
     Work w;
     std::memset(&w, 0, sizeof(w));
     w.boxes = nullptr;
+    w.cls = kCellClass;
     for (int d = 0; d < 3; ++d) {
         w.tsh[d] = c->tsh[d];
         w.one.lo[d] = bx.lo[d];
         w.one.hi[d] = bx.hi[d];
-        w.one.ntile[d] = div_up(len[d], w.tsh[d]);
+        w.one.ntile[kCellClass][d] = div_up(len[d], w.tsh[d]);
     }
     w.t0 = 0;
-    w.t1 = w.one.ntile[0] * w.one.ntile[1] * w.one.ntile[2];
+    w.t1 = box_tiles(w.one, kCellClass);
     const int NC = NCV + c->ns;
     int full = 0;
     if (flux3) {
@@ -751,7 +858,7 @@ int mf_level_create(void *ctx, const mf_box *boxes, int nboxes, void **level_out
     h.stop_time = c->prm.stop_time;
     h.active = 1;
     hipMemcpy(L->ts, &h, sizeof(h), hipMemcpyHostToDevice);
-    int st = rebuild_tiles(L, c->tsh);
+    int st = rebuild_tiles(L);
     if (st) { mf_level_destroy(L); return st; }
     *level_out = L;
     return MF_OK;
@@ -907,6 +1014,38 @@ int mf_level_step(void *level, int nsteps, int mode) {
     return h.err;
 }
 
+int mf_level_step_timed(void *level, int nsteps, int mode, double *class_ms, int *class_launches) {
+    Level *L = (Level *)level;
+    if (!L || nsteps < 0 || mode < 0 || mode > 1 || !class_ms) return MF_ERR_BAD_ARG;
+    Ctx *c = L->c;
+    if (L->need_est) {
+        enqueue_est(L);
+        L->need_est = false;
+    }
+    L->timer.on = true;
+    L->timer.ev.clear();
+    for (int s = 0; s < nsteps; ++s) enqueue_step(L, mode);
+    L->timer.on = false;
+    HIPCHK(hipGetLastError());
+    TimeState h;
+    int st = read_ts(L, &h);
+    for (int q = 0; q < KC_N; ++q) {
+        class_ms[q] = 0.0;
+        if (class_launches) class_launches[q] = 0;
+    }
+    for (auto &e : L->timer.ev) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e.second.first, e.second.second);
+        class_ms[e.first] += ms;
+        if (class_launches) class_launches[e.first] += 1;
+        hipEventDestroy(e.second.first);
+        hipEventDestroy(e.second.second);
+    }
+    L->timer.ev.clear();
+    if (st) return st;
+    return h.err;
+}
+
 int mf_level_info(void *level, double *time, double *dt, int *istep, double *resnorms) {
     Level *L = (Level *)level;
     if (!L) return MF_ERR_BAD_ARG;
@@ -935,34 +1074,43 @@ int mf_level_set_time(void *level, double time, double dt_prev, int istep) {
     return MF_OK;
 }
 
-int mf_level_set_tile(void *level, int tx, int ty, int tz) {
+int mf_level_set_tile(void *level, int cls, int tx, int ty, int tz) {
     Level *L = (Level *)level;
-    if (!L) return MF_ERR_BAD_ARG;
-    if (tx == 0 && ty == 0 && tz == 0) return mf_level_autotune(level);
+    if (!L || cls < 0 || cls >= kTileClasses) return MF_ERR_BAD_ARG;
     if (tx < 1 || ty < 1 || tz < 1 || (int64_t)tx * ty * tz > 4096) return MF_ERR_BAD_ARG;
+    if (cls < 3) {
+        int t[3] = {tx, ty, tz};
+        if (L->c->ks->lds_bytes(t, cls) > 160 * 1024) return MF_ERR_TOO_LARGE;
+    }
     HIPCHK(hipStreamSynchronize(L->c->stream));
-    int t[3] = {tx, ty, tz};
-    return rebuild_tiles(L, t);
+    L->tsh[cls][0] = tx;
+    L->tsh[cls][1] = ty;
+    L->tsh[cls][2] = tz;
+    return rebuild_tiles(L);
 }
 
-int mf_level_get_tile(void *level, int *tx, int *ty, int *tz) {
+int mf_level_get_tile(void *level, int cls, int *tx, int *ty, int *tz) {
     Level *L = (Level *)level;
-    if (!L) return MF_ERR_BAD_ARG;
-    if (tx) *tx = L->tsh[0];
-    if (ty) *ty = L->tsh[1];
-    if (tz) *tz = L->tsh[2];
+    if (!L || cls < 0 || cls >= kTileClasses) return MF_ERR_BAD_ARG;
+    if (tx) *tx = L->tsh[cls][0];
+    if (ty) *ty = L->tsh[cls][1];
+    if (tz) *tz = L->tsh[cls][2];
     return MF_OK;
 }
 
-// Autotuner: times stage 1 (ghost fill, cut cells, three face sweeps, residual
-// and combine into B) for each candidate tile shape; the state in A is only read.
+// Autotuner: for each tile class, times the kernels of that class (the face
+// sweep of one direction, or the stage-1 residual/combine) on the current state
+// for every candidate shape and keeps the fastest. The state in A is only read
+// (stage 1 writes B, the fluxes and the cut-cell buffer).
 int mf_level_autotune(void *level) {
     Level *L = (Level *)level;
     if (!L) return MF_ERR_BAD_ARG;
     Ctx *c = L->c;
-    static const int cand[][3] = {{32, 4, 2}, {64, 4, 1}, {16, 8, 2}, {32, 8, 1}, {64, 2, 2}, {16, 4, 4},
-                                  {128, 2, 1}, {8, 8, 4}, {16, 16, 1}, {32, 2, 4}};
-    const int ncand = sizeof(cand) / sizeof(cand[0]);
+    static const int face_cand[3][8][3] = {
+        {{64, 4, 1}, {64, 2, 2}, {128, 2, 1}, {32, 4, 2}, {32, 8, 1}, {64, 1, 4}, {128, 1, 2}, {32, 2, 4}},
+        {{32, 8, 1}, {16, 16, 1}, {64, 4, 1}, {32, 4, 2}, {16, 8, 2}, {64, 8, 1}, {32, 16, 1}, {16, 12, 2}},
+        {{32, 1, 8}, {16, 2, 8}, {32, 2, 4}, {64, 1, 4}, {16, 1, 16}, {64, 1, 8}, {32, 1, 16}, {16, 2, 12}}};
+    static const int cell_cand[8][3] = {{64, 2, 2}, {32, 4, 2}, {64, 4, 1}, {128, 2, 1}, {32, 8, 1}, {16, 4, 4}, {32, 2, 4}, {64, 1, 4}};
     HIPCHK(hipStreamSynchronize(c->stream));
     TimeState saved;
     int st = read_ts(L, &saved);
@@ -975,30 +1123,66 @@ int mf_level_autotune(void *level) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    double best = 1e300;
-    int bi = 0;
-    for (int q = 0; q < ncand; ++q) {
-        bool fits = true;
-        for (int d = 0; d < 3; ++d) if (cand[q][d] > 2 * L->n[d] && cand[q][d] > 1) fits = false;
-        if (!fits) continue;
-        st = rebuild_tiles(L, cand[q]);
-        if (st) break;
-        enqueue_stage(L, 1, MF_MODE_BATCHED);  // warm-up
-        HIPCHK(hipEventRecord(e0, c->stream));
-        for (int r = 0; r < 3; ++r) enqueue_stage(L, 1, MF_MODE_BATCHED);
-        HIPCHK(hipEventRecord(e1, c->stream));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) { best = ms; bi = q; }
+    // ghost cells and cut cells of stage 1 once, so the timed kernels see real data
+    {
+        FillArgs f = fill_args(L, L->arr[0], 1);
+        for (int ph = 0; ph <= 3; ++ph) c->ks->fill(f, ph, c->stream);
+        if (L->ncut) c->ks->cut(cut_args(L, L->arr[0]), c->stream);
+    }
+    StageArgs a = stage_args(L, 1);
+    ResArgs r;
+    std::memset(&r, 0, sizeof(r));
+    r.st = a;
+    r.combine = 1;
+    r.old = L->view[0];
+    r.dst = L->view[1];
+    r.do_ns = c->prm.do_ns;
+    r.partials = L->partials;
+    auto time_it = [&](auto &&launch) -> double {
+        launch();
+        hipEventRecord(e0, c->stream);
+        for (int q = 0; q < 3; ++q) launch();
+        hipEventRecord(e1, c->stream);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        return ms;
+    };
+    const bool lds = use_lds(L);
+    for (int cls = 0; cls < kTileClasses && st == MF_OK; ++cls) {
+        if (cls < 3 && !lds) continue;
+        const int ncand = 8;
+        double best = 1e300;
+        int bi[3] = {L->tsh[cls][0], L->tsh[cls][1], L->tsh[cls][2]};
+        for (int q = 0; q < ncand; ++q) {
+            const int *cand = cls < 3 ? face_cand[cls][q] : cell_cand[q];
+            bool fits = true;
+            for (int d = 0; d < 3; ++d) if (cand[d] > 2 * L->n[d] && cand[d] > 1) fits = false;
+            if (cls < 3 && c->ks->lds_bytes(cand, cls) > 96 * 1024) fits = false;
+            if (!fits) continue;
+            for (int d = 0; d < 3; ++d) L->tsh[cls][d] = cand[d];
+            st = rebuild_tiles(L);
+            if (st) break;
+            double ms;
+            if (cls < 3) ms = time_it([&] { launch_faces(L, a, cls, MF_MODE_BATCHED); });
+            else {
+                if (!lds) ms = time_it([&] {
+                    for (int d = 0; d < 3; ++d) launch_faces(L, a, d, MF_MODE_BATCHED);
+                    launch_residual(L, r, MF_MODE_BATCHED);
+                });
+                else ms = time_it([&] { launch_residual(L, r, MF_MODE_BATCHED); });
+            }
+            if (ms < best) { best = ms; bi[0] = cand[0]; bi[1] = cand[1]; bi[2] = cand[2]; }
+        }
+        for (int d = 0; d < 3; ++d) L->tsh[cls][d] = bi[d];
+        if (st == MF_OK) st = rebuild_tiles(L);
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     HIPCHK(hipMemcpy(L->ts, &saved, sizeof(saved), hipMemcpyHostToDevice));
     int ok = 0;
     HIPCHK(hipMemcpy(c->d_err, &ok, sizeof(int), hipMemcpyHostToDevice));
-    if (st) return st;
-    return rebuild_tiles(L, cand[bi]);
+    return st;
 }
 
 int mf_util_copy(void *ctx, double *dst, const double *src, int64_t n) {

@@ -62,6 +62,19 @@ struct Acc {
     MFD double operator()(int i, int j, int k, int n) const { return get(i, j, k, n, cid(i, j, k)); }
 };
 
+// All components of one cell of a box's Sborder: p[n * stride]
+struct CellPtr {
+    const double *p;
+    int64_t stride;
+    MFD double operator()(int n) const { return p[(int64_t)n * stride]; }
+};
+MFD CellPtr cell_ptr(const Acc &a, int i, int j, int k, int c) {
+    CellPtr r;
+    if (c >= 0) { r.p = a.cutbuf + c; r.stride = a.ncut; }
+    else { r.p = a.s.p + a.s.off(i, j, k); r.stride = a.s.ns; }
+    return r;
+}
+
 // One cell seen by a pointwise plugin (the reference's plugins read phi(i,j,k,*) only)
 struct CellRef {
     const Acc *a;
@@ -78,7 +91,14 @@ struct CellArr {
 // ---------------------------------------------------------------------------
 // Plugins: compile-time families with runtime tables (mf_plugin_params)
 // ---------------------------------------------------------------------------
-using Tab = mf_plugin_params;
+// Plugin table on the device: the C-ABI table plus per-species constants the
+// plugins would otherwise recompute per call (the same IEEE divisions, done once
+// on the host: cv_s = R/(gamma_s-1), gm1_s = gamma_s-1, cvm_s = R/(gamma_s-1)/molwt_s).
+struct Tab : mf_plugin_params {
+    double cv[MF_MAX_SPECIES];
+    double gm1[MF_MAX_SPECIES];
+    double cvm[MF_MAX_SPECIES];
+};
 
 // tests/Cartesian/ShockTube_N2/{thermo.H:19-133, transport.H, userfuncs.H, chemistry.H, externbc.H}
 struct PShockTubeN2 {
@@ -89,17 +109,17 @@ struct PShockTubeN2 {
         return pres / (sum_ci * T.R_univ);
     }
     MFD static double get_p_from_rec(const Tab &T, double rhoe, double rho, const double *spec) {
-        double cv = T.R_univ / (T.gamma[0] - kOne);
+        double cv = T.cv[0];
         double Tm = rhoe / (cv * spec[0]);
         return kZero + spec[0] * T.R_univ * Tm;
     }
     MFD static double get_gama_from_rpc(const Tab &T, double rho, double pres, const double *spec) {
         double sum_ci = kZero + spec[0];
         double x0 = spec[0] / sum_ci;
-        return kOne + kOne / (x0 / (T.gamma[0] - kOne));
+        return kOne + kOne / (x0 / T.gm1[0]);
     }
     MFD static double get_e_from_rpc(const Tab &T, double rho, double pres, const double *spec) {
-        double cv = T.R_univ / (T.gamma[0] - kOne) / T.molwt[0];
+        double cv = T.cvm[0];
         double sum_ci = kZero + spec[0];
         double x0 = spec[0] / sum_ci;
         double temp = get_t_from_rpc(T, rho, pres, spec);
@@ -115,6 +135,7 @@ struct PShockTubeN2 {
     }
     static constexpr bool kHasFluidSource = false;
     static constexpr bool kHasChemistry = false;
+    static constexpr bool kGammaIgnoresRhoP = true;  // gamma depends on the species only
     template <class C> MFD static void spec_source(const Tab &, const C &, double *src) { src[0] = kZero; }
     MFD static void externalbc(const Tab &T, const double *x, const double *s_int, double *s_ext, int idir, int sgn,
                                double time) {
@@ -150,7 +171,7 @@ struct PMixture {
     MFD static double get_p_from_rec(const Tab &T, double rhoe, double rho, const double *spec) {
         double cvs = kZero;
 #pragma unroll
-        for (int s = 0; s < NG; ++s) cvs += (T.R_univ / (T.gamma[s] - kOne)) * spec[s];
+        for (int s = 0; s < NG; ++s) cvs += T.cv[s] * spec[s];
         double Tm = rhoe / cvs;
         double p = kZero;
 #pragma unroll
@@ -163,14 +184,14 @@ struct PMixture {
         for (int s = 0; s < NG; ++s) sum_ci += spec[s];
         double term2 = kZero;
 #pragma unroll
-        for (int s = 0; s < NG; ++s) term2 += (spec[s] / sum_ci) / (T.gamma[s] - kOne);
+        for (int s = 0; s < NG; ++s) term2 += (spec[s] / sum_ci) / T.gm1[s];
         return kOne + kOne / term2;
     }
     MFD static double get_e_from_rpc(const Tab &T, double rho, double pres, const double *spec) {
         double temp = get_t_from_rpc(T, rho, pres, spec);
         double cvc = kZero;
 #pragma unroll
-        for (int s = 0; s < NG; ++s) cvc += (T.R_univ / (T.gamma[s] - kOne)) * spec[s];
+        for (int s = 0; s < NG; ++s) cvc += T.cv[s] * spec[s];
         return cvc * temp / rho;
     }
     MFD static double get_solid_t_from_rhoe(const Tab &, double) { return 300.0; }
@@ -189,6 +210,7 @@ struct PMixture {
         for (int n = 0; n < FLO; ++n) src[n] = kZero;
     }
     static constexpr bool kHasFluidSource = false;
+    static constexpr bool kGammaIgnoresRhoP = true;  // gamma depends on the species only
 };
 
 // models/catalystCylinder: chemistry.H:13-29, externbc.H:13-73
@@ -441,48 +463,58 @@ struct Num {
         fh[RHOU + D] += p_half;
     }
     // WENO5-JS half of flux_schemes.H:400-421 (left-biased value at the face
-    // between s2 and s3) and :423-444 (right-biased value), eps = 1e-6
-    MFD static double weno_plus(double s0, double s1, double s2, double s3, double s4) {
+    // between s2 and s3) and :423-444 (right-biased value), eps = 1e-6, p = 2,
+    // linear weights (0.1, 0.6, 0.3). With q_k = (eps + beta_k)^2 the nonlinear
+    // weights alpha_k = d_k / q_k are normalised with one division:
+    //   sum_k alpha_k v_k / sum_k alpha_k
+    //     = (v2 q1 q0 + 6 v1 q2 q0 + 3 v0 q2 q1) / (q1 q0 + 6 q2 q0 + 3 q2 q1)
+    // (the reference forms 1/q_k and 1/sum alpha; the value is the same to
+    // rounding). Building with MF_WENO_REFERENCE_DIVISIONS restores the
+    // reference's four divisions.
+    MFD static double weno_combine(double b2, double b1, double b0, double v2, double v1, double v0) {
         const double eps = 1e-6;
-        double t, b2, b1, b0;
-        t = s0 - 2.0 * s1 + s2;
+#ifdef MF_WENO_REFERENCE_DIVISIONS
+        b2 = 1.0 / ((eps + b2) * (eps + b2));
+        b1 = 1.0 / ((eps + b1) * (eps + b1));
+        b0 = 1.0 / ((eps + b0) * (eps + b0));
+        double a2 = b2, a1 = 6.0 * b1, a0 = 3.0 * b0;
+        double alpha1 = 1.0 / (a2 + a1 + a0);
+        return (1.0 / 6.0) * alpha1 * (a2 * v2 + a1 * v1 + a0 * v0);
+#else
+        const double q2 = (eps + b2) * (eps + b2);
+        const double q1 = (eps + b1) * (eps + b1);
+        const double q0 = (eps + b0) * (eps + b0);
+        const double w2 = q1 * q0, w1 = 6.0 * (q2 * q0), w0 = 3.0 * (q2 * q1);
+        return (w2 * v2 + w1 * v1 + w0 * v0) / (6.0 * (w2 + w1 + w0));
+#endif
+    }
+    MFD static double weno_plus(double s0, double s1, double s2, double s3, double s4) {
+        double t = s0 - 2.0 * s1 + s2;
         double q = s0 - 4.0 * s1 + 3.0 * s2;
-        b2 = (13.0 / 12.0) * (t * t) + 0.25 * (q * q);
+        const double b2 = (13.0 / 12.0) * (t * t) + 0.25 * (q * q);
         t = s1 - 2.0 * s2 + s3;
-        b1 = (13.0 / 12.0) * (t * t) + 0.25 * (s1 - s3) * (s1 - s3);
+        const double b1 = (13.0 / 12.0) * (t * t) + 0.25 * (s1 - s3) * (s1 - s3);
         t = s2 - 2.0 * s3 + s4;
         q = 3.0 * s2 - 4.0 * s3 + s4;
-        b0 = (13.0 / 12.0) * (t * t) + 0.25 * (q * q);
-        b2 = 1.0 / ((eps + b2) * (eps + b2));
-        b1 = 1.0 / ((eps + b1) * (eps + b1));
-        b0 = 1.0 / ((eps + b0) * (eps + b0));
-        double a2 = b2, a1 = 6.0 * b1, a0 = 3.0 * b0;
-        double alpha1 = 1.0 / (a2 + a1 + a0);
-        double v2 = 2.0 * s0 - 7.0 * s1 + 11.0 * s2;
-        double v1 = -s1 + 5.0 * s2 + 2.0 * s3;
-        double v0 = 2.0 * s2 + 5.0 * s3 - s4;
-        return (1.0 / 6.0) * alpha1 * (a2 * v2 + a1 * v1 + a0 * v0);
+        const double b0 = (13.0 / 12.0) * (t * t) + 0.25 * (q * q);
+        const double v2 = 2.0 * s0 - 7.0 * s1 + 11.0 * s2;
+        const double v1 = -s1 + 5.0 * s2 + 2.0 * s3;
+        const double v0 = 2.0 * s2 + 5.0 * s3 - s4;
+        return weno_combine(b2, b1, b0, v2, v1, v0);
     }
     MFD static double weno_minus(double s1, double s2, double s3, double s4, double s5) {
-        const double eps = 1e-6;
-        double t, q, b2, b1, b0;
-        t = s5 - 2.0 * s4 + s3;
-        q = s5 - 4.0 * s4 + 3.0 * s3;
-        b2 = (13.0 / 12.0) * (t * t) + 0.25 * (q * q);
+        double t = s5 - 2.0 * s4 + s3;
+        double q = s5 - 4.0 * s4 + 3.0 * s3;
+        const double b2 = (13.0 / 12.0) * (t * t) + 0.25 * (q * q);
         t = s4 - 2.0 * s3 + s2;
-        b1 = (13.0 / 12.0) * (t * t) + 0.25 * (s4 - s2) * (s4 - s2);
+        const double b1 = (13.0 / 12.0) * (t * t) + 0.25 * (s4 - s2) * (s4 - s2);
         t = s3 - 2.0 * s2 + s1;
         q = 3.0 * s3 - 4.0 * s2 + s1;
-        b0 = (13.0 / 12.0) * (t * t) + 0.25 * (q * q);
-        b2 = 1.0 / ((eps + b2) * (eps + b2));
-        b1 = 1.0 / ((eps + b1) * (eps + b1));
-        b0 = 1.0 / ((eps + b0) * (eps + b0));
-        double a2 = b2, a1 = 6.0 * b1, a0 = 3.0 * b0;
-        double alpha1 = 1.0 / (a2 + a1 + a0);
-        double v2 = 11.0 * s3 - 7.0 * s4 + 2.0 * s5;
-        double v1 = -s4 + 5.0 * s3 + 2.0 * s2;
-        double v0 = 2.0 * s3 + 5.0 * s2 - s1;
-        return (1.0 / 6.0) * alpha1 * (a2 * v2 + a1 * v1 + a0 * v0);
+        const double b0 = (13.0 / 12.0) * (t * t) + 0.25 * (q * q);
+        const double v2 = 11.0 * s3 - 7.0 * s4 + 2.0 * s5;
+        const double v1 = -s4 + 5.0 * s3 + 2.0 * s2;
+        const double v0 = 2.0 * s3 + 5.0 * s2 - s1;
+        return weno_combine(b2, b1, b0, v2, v1, v0);
     }
     // compute_flux_3d.H:16-22
     MFD static double harmonic_mean(double d1, double d2) { return 2.0 * d1 * d2 / (d1 + d2 + 1e-30); }

@@ -16,22 +16,47 @@ MFD void raise_err(int *err, int code) {
     if (err) atomicCAS(err, 0, code);
 }
 
+// Tiles of a launch: with a grid that is a multiple of 8, block b runs on XCD
+// b % 8 (round-robin dispatch); each XCD then walks its own contiguous eighth of
+// the tile list, so tiles that share halo cells share that XCD's L2.
+struct TileIter {
+    int t, step, end;
+    MFD TileIter(const Work &w) {
+        const int G = gridDim.x, b = blockIdx.x, n = w.t1 - w.t0;
+        if ((G & 7) == 0 && n >= G) {
+            const int g = b & 7, r = b >> 3, per = G >> 3;
+            const int chunk = (n + 7) >> 3;
+            const int s0 = w.t0 + g * chunk;
+            end = min(w.t1, s0 + chunk);
+            t = s0 + r;
+            step = per;
+        } else {
+            t = w.t0 + b;
+            step = G;
+            end = w.t1;
+        }
+    }
+    MFD bool ok() const { return t < end; }
+    MFD void next() { t += step; }
+};
+
 // tile t of the work list -> its box and inclusive cell range
 MFD const BoxDesc &tile_box(const Work &w, int t, int &lt) {
     if (!w.boxes) { lt = t; return w.one; }
     int lo = 0, hi = w.nboxes - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (w.boxes[mid].tile_start <= t) lo = mid; else hi = mid - 1;
+        if (w.boxes[mid].tile_start[w.cls] <= t) lo = mid; else hi = mid - 1;
     }
-    lt = t - w.boxes[lo].tile_start;
+    lt = t - w.boxes[lo].tile_start[w.cls];
     return w.boxes[lo];
 }
 MFD void tile_range(const BoxDesc &b, const Work &w, int lt, int *c0, int *c1) {
-    int ti = lt % b.ntile[0];
-    int r = lt / b.ntile[0];
-    int tj = r % b.ntile[1];
-    int tk = r / b.ntile[1];
+    const int *nt = b.ntile[w.cls];
+    int ti = lt % nt[0];
+    int r = lt / nt[0];
+    int tj = r % nt[1];
+    int tk = r / nt[1];
     int tt[3] = {ti, tj, tk};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -55,250 +80,265 @@ MFD Acc make_acc(const StageArgs &a, const BoxDesc &b) {
 // flux component c (0..4 conserved, 5.. species) -> component of the flux view
 MFD int flux_comp(int c, int full) { return full ? (c < NCV ? c : c + (FLO - NCV)) : c; }
 
+// pass 2 of a face: viscous stress and work (:293-367), conduction (:373-374),
+// species diffusion (:425-428), applied to the stored hyperbolic part
+template <class P, int D>
+MFD void face_pass2(const Tab &T, const Acc &acc, const StageArgs &a, int i, int j, int k, const View &F,
+                    const CellPtr &pL, const CellPtr &pR, int cL, int cR, bool hyp) {
+    using N = Num<P>;
+    constexpr int NS = P::NS;
+    constexpr int T1 = (D + 1) % 3, T2 = (D + 2) % 3;
+    constexpr int e0 = D == 0, e1 = D == 1, e2 = D == 2;
+    const int iL = i - e0, jL = j - e1, kL = k - e2;
+    const double dxs = a.dx[D];
+    const bool ns = a.nsflag != 0;
+    const int full = a.flux_full;
+    double *Fp = F.p + F.off(i, j, k);
+    const int64_t Fs = F.ns;
+    auto fput = [&](int c, double v) { Fp[(int64_t)flux_comp(c, full) * Fs] = v; };
+    auto fget = [&](int c) { return Fp[(int64_t)flux_comp(c, full) * Fs]; };
+    if (ns) {
+        double visc, thcond;
+        {
+            // harmonic mean of (i,j,k) and (i-1,j,k) for every sweep direction (:179-180)
+            double v0, k0, v1, k1;
+            CellRef c0{&acc, i, j, k, cR};
+            P::fluid_transport(T, c0, v0, k0);
+            CellRef c1{&acc, i - 1, j, k, acc.cid(i - 1, j, k)};
+            P::fluid_transport(T, c1, v1, k1);
+            visc = N::harmonic_mean(v0, v1);
+            thcond = N::harmonic_mean(k0, k1);
+        }
+        if (hyp) {
+            double g[3][3];  // g[q][m]: d(vel q) along normal (m=0), t1 (m=1), t2 (m=2)
+            constexpr int p10 = T1 == 0, p11 = T1 == 1, p12 = T1 == 2;
+            constexpr int p20 = T2 == 0, p21 = T2 == 1, p22 = T2 == 2;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int V = VELX + q;
+                g[q][0] = (pR(V) - pL(V)) / dxs;
+                double top = kHalf * (acc(iL + p10, jL + p11, kL + p12, V) + acc(i + p10, j + p11, k + p12, V));
+                double bot = kHalf * (acc(iL - p10, jL - p11, kL - p12, V) + acc(i - p10, j - p11, k - p12, V));
+                g[q][1] = (top - bot) / (kTwo * a.dx[T1]);
+                double frt = kHalf * (acc(iL + p20, jL + p21, kL + p22, V) + acc(i + p20, j + p21, k + p22, V));
+                double bck = kHalf * (acc(iL - p20, jL - p21, kL - p22, V) + acc(i - p20, j - p21, k - p22, V));
+                g[q][2] = (frt - bck) / (kTwo * a.dx[T2]);
+            }
+            // axis -> gradient slot (GET_XDIR/GET_YDIR/GET_ZDIR, globalDefines.H:49-51)
+            constexpr int sx = D == 0 ? 0 : (T1 == 0 ? 1 : 2);
+            constexpr int sy = D == 1 ? 0 : (T1 == 1 ? 1 : 2);
+            constexpr int sz = D == 2 ? 0 : (T1 == 2 ? 1 : 2);
+            double dudx = g[0][sx], dudy = g[0][sy], dudz = g[0][sz];
+            double dvdx = g[1][sx], dvdy = g[1][sy], dvdz = g[1][sz];
+            double dwdx = g[2][sx], dwdy = g[2][sy], dwdz = g[2][sz];
+            double delu = dudx + dvdy + dwdz;
+            double t0, t1, t2;
+            if (D == 0) {
+                t0 = visc * (kTwo * dudx - kTwo3rd * delu);
+                t1 = visc * (dudy + dvdx);
+                t2 = visc * (dudz + dwdx);
+            } else if (D == 1) {
+                t0 = visc * (dvdx + dudy);
+                t1 = visc * (kTwo * dvdy - kTwo3rd * delu);
+                t2 = visc * (dvdz + dwdy);
+            } else {
+                t0 = visc * (dwdx + dudz);
+                t1 = visc * (dwdy + dvdz);
+                t2 = visc * (kTwo * dwdz - kTwo3rd * delu);
+            }
+            fput(RHOU, fget(RHOU) - t0);
+            fput(RHOV, fget(RHOV) - t1);
+            fput(RHOW, fget(RHOW) - t2);
+            double e = fget(RHOE);
+            e -= t0 * (kHalf * (pL(VELX) + pR(VELX)));
+            e -= t1 * (kHalf * (pL(VELY) + pR(VELY)));
+            e -= t2 * (kHalf * (pL(VELZ) + pR(VELZ)));
+            fput(RHOE, e);
+        }
+        double dTdn = (pR(TEMP) - pL(TEMP)) / dxs;
+        fput(RHOE, fget(RHOE) - thcond * dTdn);
+    }
+    {
+        double dl[NS], dr[NS];
+        CellRef l{&acc, iL, jL, kL, cL};
+        CellRef r{&acc, i, j, k, cR};
+        P::spec_dcoeff(T, l, dl);
+        P::spec_dcoeff(T, r, dr);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double Ds = N::harmonic_mean(dl[s], dr[s]);
+            fput(NCV + s, fget(NCV + s) - Ds * (pR(FLO + s) - pL(FLO + s)) / dxs);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Face flux of direction D at face (i,j,k) (between (i,j,k)-e_D and (i,j,k)):
-// Source/Src_3d/compute_flux_3d.H:86-430.
+// Source/Src_3d/compute_flux_3d.H:86-430. Pass 1 stores the hyperbolic (or
+// convective) part of every flux component; pass 2 applies the viscous stress
+// and work, conduction and species diffusion to the stored values in the
+// reference's order, so the stencil data and the viscous data are never live
+// at the same time.
 // ---------------------------------------------------------------------------
 template <class P, int ORDER, int D>
 MFD void face_flux(const Tab &T, const Acc &acc, const StageArgs &a, int i, int j, int k, const View &F, double time) {
     using N = Num<P>;
     constexpr int NS = P::NS, NC = N::NC;
     constexpr int T1 = (D + 1) % 3, T2 = (D + 2) % 3;
-    int e[3] = {0, 0, 0};
-    e[D] = 1;
-    // stencil cell s (s = -3..2) along D; -1 is `left`, 0 is `right`
-    auto ci = [&](int s) { return i + s * e[0]; };
-    auto cj = [&](int s) { return j + s * e[1]; };
-    auto ck = [&](int s) { return k + s * e[2]; };
-    int cid[6];
-    double vf[6];
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        cid[s] = acc.cid(ci(s - 3), cj(s - 3), ck(s - 3));
-        vf[s] = acc.get(ci(s - 3), cj(s - 3), ck(s - 3), VFRAC, cid[s]);
-    }
-    const int cL = cid[2], cR = cid[3];
-    const int iL = ci(-1), jL = cj(-1), kL = ck(-1);
-    // viscosity / conductivity: harmonic mean of (i,j,k) and (i-1,j,k) for every
-    // sweep direction (compute_flux_3d.H:179-180)
-    double visc, thcond;
-    {
-        double v0, k0, v1, k1;
-        CellRef c0{&acc, i, j, k, cR};
-        P::fluid_transport(T, c0, v0, k0);
-        CellRef c1{&acc, i - 1, j, k, acc.cid(i - 1, j, k)};
-        P::fluid_transport(T, c1, v1, k1);
-        visc = N::harmonic_mean(v0, v1);
-        thcond = N::harmonic_mean(k0, k1);
-    }
-    double dl[NS], dr[NS];
-    {
-        CellRef l{&acc, iL, jL, kL, cL};
-        CellRef r{&acc, i, j, k, cR};
-        P::spec_dcoeff(T, l, dl);
-        P::spec_dcoeff(T, r, dr);
-    }
-    double facevel[3];
-    facevel[0] = kHalf * (acc.get(iL, jL, kL, VELX, cL) + acc.get(i, j, k, VELX, cR));
-    facevel[1] = kHalf * (acc.get(iL, jL, kL, VELY, cL) + acc.get(i, j, k, VELY, cR));
-    facevel[2] = kHalf * (acc.get(iL, jL, kL, VELZ, cL) + acc.get(i, j, k, VELZ, cR));
-    const double facevfrac = kHalf * (vf[2] + vf[3]);
+    constexpr int e0 = D == 0, e1 = D == 1, e2 = D == 2;
+    const int iL = i - e0, jL = j - e1, kL = k - e2;
+    const int cL = acc.cid(iL, jL, kL), cR = acc.cid(i, j, k);
+    const CellPtr pL = cell_ptr(acc, iL, jL, kL, cL), pR = cell_ptr(acc, i, j, k, cR);
+    const double facevfrac = kHalf * (pL(VFRAC) + pR(VFRAC));
     const double dxs = a.dx[D];
-    double Fo[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) Fo[c] = kZero;
+    const bool ns = a.nsflag != 0;
+    const bool hyp = ns && facevfrac >= 0.5;
+    const int full = a.flux_full;
+    double *Fp = F.p + F.off(i, j, k);
+    const int64_t Fs = F.ns;
+    auto fput = [&](int c, double v) { Fp[(int64_t)flux_comp(c, full) * Fs] = v; };
+    auto fget = [&](int c) { return Fp[(int64_t)flux_comp(c, full) * Fs]; };
 
-    if (a.nsflag) {
-        const bool first = (vf[0] < kOne || vf[1] < kOne || vf[2] < kOne || vf[3] < kOne || vf[4] < kOne || vf[5] < kOne);
-        if (facevfrac >= 0.5) {
-            double fh[NC];
-            // stencil positions after the IB first-order fallback (:233-249)
-            const int sidx[6] = {first ? -1 : -3, first ? -1 : -2, -1, 0, first ? 0 : 1, first ? 0 : 2};
-            auto load = [&](int s, double *u) {
-                int q = s + 3;
-                int ii = ci(s), jj = cj(s), kk = ck(s);
-                int c = (s == -1) ? cL : ((s == 0) ? cR : cid[q]);
-#pragma unroll
-                for (int n = 0; n < NCV; ++n) u[n] = acc.get(ii, jj, kk, n, c);
-#pragma unroll
-                for (int n = 0; n < NS; ++n) u[NCV + n] = acc.get(ii, jj, kk, FLO + n, c);
-            };
-            if (ORDER == 1 || ORDER == 2) {
-                double ul[NC], ur[NC];
-                load(-1, ul);
-                load(0, ur);
-                if (ORDER == 2) {
-                    // compute_flux_3d.H:31-65 with uim2 = ulm1, uip1 = urp1
-                    double ulm1[NC], urp1[NC];
-                    load(sidx[1], ulm1);
-                    load(sidx[4], urp1);
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) {
-                        double d1 = urp1[c] - ur[c];
-                        double d2 = ul[c] - ulm1[c];
-                        double r = fabs((d1 - d2) / (fabs(d1) + fabs(d2) + kTiny));
-                        double psi = kOne - r * r;
-                        double L = kHalf * psi * (d1 + d2);
-                        ul[c] = ul[c] + kHalf * L;
-                        ur[c] = ur[c] - kHalf * L;
-                    }
-                }
-                N::template ausmp_up<D>(T, ul, ur, fh);
-            } else if (ORDER == 4) {
-                // flux_schemes.H:312-361 with uim1 = ul, ui = ur, uim2 = ulm1, uip1 = urp1
-                double um2[NC], um1[NC], u0[NC], up1[NC];
-                load(sidx[1], um2);
-                load(-1, um1);
-                load(0, u0);
-                load(sidx[4], up1);
-                double pv[4][4];
-                N::euler_prims(T, um2, pv[0][0], pv[0][1], pv[0][2], pv[0][3]);
-                N::euler_prims(T, um1, pv[1][0], pv[1][1], pv[1][2], pv[1][3]);
-                N::euler_prims(T, u0, pv[2][0], pv[2][1], pv[2][2], pv[2][3]);
-                N::euler_prims(T, up1, pv[3][0], pv[3][1], pv[3][2], pv[3][3]);
-                double rhoL = um1[RHO], rhoR = u0[RHO];
-                double rhoeL = um1[RHOE] - kHalf * (um1[RHOU] * um1[RHOU] + um1[RHOV] * um1[RHOV] + um1[RHOW] * um1[RHOW]) / um1[RHO];
-                double rhoeR = u0[RHOE] - kHalf * (u0[RHOU] * u0[RHOU] + u0[RHOV] * u0[RHOV] + u0[RHOW] * u0[RHOW]) / u0[RHO];
-                double presL = P::get_p_from_rec(T, rhoeL, um1[RHO], um1 + NCV);
-                double presR = P::get_p_from_rec(T, rhoeR, u0[RHO], u0 + NCV);
-                double veln = fmax(fabs(um1[RHOU + D] / um1[RHO]), fabs(u0[RHOU + D] / u0[RHO]));
-                double gamaL = P::get_gama_from_rpc(T, rhoL, presL, um1 + NCV);
-                double gamaR = P::get_gama_from_rpc(T, rhoR, presR, u0 + NCV);
-                double ahalf = fmax(sqrt(gamaL * presL / rhoL), sqrt(gamaR * presR / rhoR));
-#pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    double fm2 = N::template euler_comp<D>(c, um2, pv[0][0], pv[0][1], pv[0][2], pv[0][3]);
-                    double fm1 = N::template euler_comp<D>(c, um1, pv[1][0], pv[1][1], pv[1][2], pv[1][3]);
-                    double f0 = N::template euler_comp<D>(c, u0, pv[2][0], pv[2][1], pv[2][2], pv[2][3]);
-                    double fp1 = N::template euler_comp<D>(c, up1, pv[3][0], pv[3][1], pv[3][2], pv[3][3]);
-                    double v = kTwelveth * (7 * f0 + 7 * fm1 - fm2 - fp1);
-                    v -= kTwelveth * (3 * u0[c] - 3 * um1[c] - up1[c] + u0[c]) * (veln + ahalf) * a.dissfactor;
-                    fh[c] = v;
-                }
-            } else {
-                // WENO5 flux splitting, flux_schemes.H:448-495
-                double lam;
-                {
-                    double ul[NC], ur[NC];
-                    load(-1, ul);
-                    load(0, ur);
-                    lam = N::template max_eigenvalue<D>(T, ul, ur);
-                }
-                double pr[6][5];  // rho, vx, vy, vz, p of the six stencil cells
-#pragma unroll
-                for (int s = 0; s < 6; ++s) {
-                    double u[NC];
-                    load(sidx[s], u);
-                    pr[s][0] = u[RHO];
-                    N::euler_prims(T, u, pr[s][1], pr[s][2], pr[s][3], pr[s][4]);
-                }
-                int sc[6];
-#pragma unroll
-                for (int s = 0; s < 6; ++s) sc[s] = (sidx[s] == -1) ? cL : ((sidx[s] == 0) ? cR : cid[sidx[s] + 3]);
-#pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    const int comp = c < NCV ? c : FLO + (c - NCV);
-                    double uc[6], fc[6];
-#pragma unroll
-                    for (int s = 0; s < 6; ++s) {
-                        int ss = sidx[s];
-                        uc[s] = acc.get(ci(ss), cj(ss), ck(ss), comp, sc[s]);
-                        double vn = pr[s][1 + D];
-                        if (c == RHO) fc[s] = pr[s][0] * vn;
-                        else if (c >= RHOU && c <= RHOW) {
-                            double m = pr[s][0] * pr[s][c];
-                            fc[s] = (c - RHOU == D) ? (m * pr[s][c] + pr[s][4]) : (m * vn);
-                        } else if (c == RHOE) fc[s] = (uc[s] + pr[s][4]) * vn;
-                        else fc[s] = uc[s] * vn;
-                    }
-                    double fp = N::weno_plus(fc[0], fc[1], fc[2], fc[3], fc[4]);
-                    double fm = N::weno_minus(fc[1], fc[2], fc[3], fc[4], fc[5]);
-                    double up = N::weno_plus(uc[0], uc[1], uc[2], uc[3], uc[4]);
-                    double um = N::weno_minus(uc[1], uc[2], uc[3], uc[4], uc[5]);
-                    fh[c] = kHalf * (fp + lam * up) + kHalf * (fm - lam * um);
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < NCV; ++c) Fo[c] = fh[c];
-#pragma unroll
-            for (int s = 0; s < NS; ++s) Fo[NCV + s] = T.advect_flag[s] * fh[NCV + s];
-
-            // viscous stress and work (:293-367); the unit normal keeps row D of tau
-            double g[3][3];  // g[q][m]: derivative of velocity q along normal (m=0), t1 (m=1), t2 (m=2)
-            {
-                int tl[3] = {iL, jL, kL}, tr[3] = {i, j, k};
-                int p1[3] = {0, 0, 0}, p2[3] = {0, 0, 0};
-                p1[T1] = 1;
-                p2[T2] = 1;
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    int V = VELX + q;
-                    double vr = acc.get(i, j, k, V, cR), vl = acc.get(iL, jL, kL, V, cL);
-                    g[q][0] = (vr - vl) / dxs;
-                    double top = kHalf * (acc(tl[0] + p1[0], tl[1] + p1[1], tl[2] + p1[2], V) + acc(tr[0] + p1[0], tr[1] + p1[1], tr[2] + p1[2], V));
-                    double bot = kHalf * (acc(tl[0] - p1[0], tl[1] - p1[1], tl[2] - p1[2], V) + acc(tr[0] - p1[0], tr[1] - p1[1], tr[2] - p1[2], V));
-                    g[q][1] = (top - bot) / (kTwo * a.dx[T1]);
-                    double frt = kHalf * (acc(tl[0] + p2[0], tl[1] + p2[1], tl[2] + p2[2], V) + acc(tr[0] + p2[0], tr[1] + p2[1], tr[2] + p2[2], V));
-                    double bck = kHalf * (acc(tl[0] - p2[0], tl[1] - p2[1], tl[2] - p2[2], V) + acc(tr[0] - p2[0], tr[1] - p2[1], tr[2] - p2[2], V));
-                    g[q][2] = (frt - bck) / (kTwo * a.dx[T2]);
-                }
-            }
-            // axis a -> slot of the gradient array (GET_XDIR/GET_YDIR/GET_ZDIR)
-            auto slot = [&](int ax) { return ax == D ? 0 : (ax == T1 ? 1 : 2); };
-            double dudx = g[0][slot(0)], dudy = g[0][slot(1)], dudz = g[0][slot(2)];
-            double dvdx = g[1][slot(0)], dvdy = g[1][slot(1)], dvdz = g[1][slot(2)];
-            double dwdx = g[2][slot(0)], dwdy = g[2][slot(1)], dwdz = g[2][slot(2)];
-            double delu = dudx + dvdy + dwdz;
-            double tau0, tau1, tau2;  // row D of tau
-            if (D == 0) {
-                tau0 = visc * (kTwo * dudx - kTwo3rd * delu);
-                tau1 = visc * (dudy + dvdx);
-                tau2 = visc * (dudz + dwdx);
-            } else if (D == 1) {
-                tau0 = visc * (dvdx + dudy);
-                tau1 = visc * (kTwo * dvdy - kTwo3rd * delu);
-                tau2 = visc * (dvdz + dwdy);
-            } else {
-                tau0 = visc * (dwdx + dudz);
-                tau1 = visc * (dwdy + dvdz);
-                tau2 = visc * (kTwo * dwdz - kTwo3rd * delu);
-            }
-            Fo[RHOU] -= tau0;
-            Fo[RHOV] -= tau1;
-            Fo[RHOW] -= tau2;
-            Fo[RHOE] -= tau0 * facevel[0];
-            Fo[RHOE] -= tau1 * facevel[1];
-            Fo[RHOE] -= tau2 * facevel[2];
-        }
-        double dTdn = (acc.get(i, j, k, TEMP, cR) - acc.get(iL, jL, kL, TEMP, cL)) / dxs;
-        Fo[RHOE] -= thcond * dTdn;
-    } else {
+    // ---- pass 1: hyperbolic / convective part ----
+    if (!ns) {
         // species convection without the flow solve (:379-422)
+        double velface = kHalf * (pL(VELX + D) + pR(VELX + D));
+        if (a.spec_in_solid) {
+            double w = (pL(VFRAC) < kOne) ? kZero : kOne;  // both weights test the left cell
+            velface = (w * pL(VELX + D) + w * pR(VELX + D)) / (w + w + kTiny);
+        }
 #pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            double phiL = acc.get(iL, jL, kL, FLO + s, cL);
-            double phiR = acc.get(i, j, k, FLO + s, cR);
-            double velface = facevel[D];
-            if (a.spec_in_solid) {
-                double w = (vf[2] < kOne) ? kZero : kOne;  // both weights test the left cell
-                velface = (w * acc.get(iL, jL, kL, VELX + D, cL) + w * acc.get(i, j, k, VELX + D, cR)) / (w + w + kTiny);
+        for (int c = 0; c < NCV; ++c) fput(c, kZero);
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            fput(NCV + s, T.advect_flag[s] * (kHalf * (velface + fabs(velface)) * pL(FLO + s) +
+                                              kHalf * (velface - fabs(velface)) * pR(FLO + s)));
+    } else if (!hyp) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) fput(c, kZero);
+    } else {
+        // stencil along D: q = 0..5 is offset q-3; IB first-order fallback (:233-249)
+        CellPtr cp[6];
+        bool first = false;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            if (q == 2) cp[q] = pL;
+            else if (q == 3) cp[q] = pR;
+            else {
+                const int ii = i + (q - 3) * e0, jj = j + (q - 3) * e1, kk = k + (q - 3) * e2;
+                cp[q] = cell_ptr(acc, ii, jj, kk, acc.cid(ii, jj, kk));
             }
-            Fo[NCV + s] = T.advect_flag[s] * (kHalf * (velface + fabs(velface)) * phiL + kHalf * (velface - fabs(velface)) * phiR);
+            first = first || (cp[q](VFRAC) < kOne);
+        }
+        if (first) { cp[0] = pL; cp[1] = pL; cp[4] = pR; cp[5] = pR; }
+        auto load = [&](const CellPtr &c, double *u) {
+#pragma unroll
+            for (int n = 0; n < NCV; ++n) u[n] = c(n);
+#pragma unroll
+            for (int n = 0; n < NS; ++n) u[NCV + n] = c(FLO + n);
+        };
+        auto adv = [&](int c, double h) { return c < NCV ? h : T.advect_flag[c - NCV] * h; };
+        if (ORDER == 5) {
+            // WENO5 flux splitting, flux_schemes.H:448-495
+            double lam;
+            {
+                double ul[NC], ur[NC];
+                load(pL, ul);
+                load(pR, ur);
+                lam = N::template max_eigenvalue<D>(T, ul, ur);
+            }
+            double pr[6][5];  // rho, vx, vy, vz, p of the six stencil cells
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                double u[NC];
+                load(cp[q], u);
+                pr[q][0] = u[RHO];
+                N::euler_prims(T, u, pr[q][1], pr[q][2], pr[q][3], pr[q][4]);
+            }
+#pragma unroll 1
+            for (int c = 0; c < NC; ++c) {
+                const int comp = c < NCV ? c : FLO + (c - NCV);
+                double uc[6], fc[6];
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    uc[q] = cp[q](comp);
+                    const double vn = pr[q][1 + D];
+                    if (c == RHO) fc[q] = pr[q][0] * vn;
+                    else if (c >= RHOU && c <= RHOW) {
+                        const double va = (c == RHOU) ? pr[q][1] : ((c == RHOV) ? pr[q][2] : pr[q][3]);
+                        const double m = pr[q][0] * va;
+                        fc[q] = (c - RHOU == D) ? (m * va + pr[q][4]) : (m * vn);
+                    } else if (c == RHOE) fc[q] = (uc[q] + pr[q][4]) * vn;
+                    else fc[q] = uc[q] * vn;
+                }
+                double fp = N::weno_plus(fc[0], fc[1], fc[2], fc[3], fc[4]);
+                double fm = N::weno_minus(fc[1], fc[2], fc[3], fc[4], fc[5]);
+                double up = N::weno_plus(uc[0], uc[1], uc[2], uc[3], uc[4]);
+                double um = N::weno_minus(uc[1], uc[2], uc[3], uc[4], uc[5]);
+                fput(c, adv(c, kHalf * (fp + lam * up) + kHalf * (fm - lam * um)));
+            }
+        } else if (ORDER == 1 || ORDER == 2) {
+            double ul[NC], ur[NC], fh[NC];
+            load(pL, ul);
+            load(pR, ur);
+            if (ORDER == 2) {
+                // compute_flux_3d.H:31-65 with uim2 = ulm1, uip1 = urp1
+                double ulm1[NC], urp1[NC];
+                load(cp[1], ulm1);
+                load(cp[4], urp1);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    double d1 = urp1[c] - ur[c];
+                    double d2 = ul[c] - ulm1[c];
+                    double r = fabs((d1 - d2) / (fabs(d1) + fabs(d2) + kTiny));
+                    double psi = kOne - r * r;
+                    double L = kHalf * psi * (d1 + d2);
+                    ul[c] = ul[c] + kHalf * L;
+                    ur[c] = ur[c] - kHalf * L;
+                }
+            }
+            N::template ausmp_up<D>(T, ul, ur, fh);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) fput(c, adv(c, fh[c]));
+        } else {
+            // flux_schemes.H:312-361 with uim1 = ul, ui = ur, uim2 = ulm1, uip1 = urp1
+            double um2[NC], um1[NC], u0[NC], up1[NC];
+            load(cp[1], um2);
+            load(pL, um1);
+            load(pR, u0);
+            load(cp[4], up1);
+            double pv[4][4];
+            N::euler_prims(T, um2, pv[0][0], pv[0][1], pv[0][2], pv[0][3]);
+            N::euler_prims(T, um1, pv[1][0], pv[1][1], pv[1][2], pv[1][3]);
+            N::euler_prims(T, u0, pv[2][0], pv[2][1], pv[2][2], pv[2][3]);
+            N::euler_prims(T, up1, pv[3][0], pv[3][1], pv[3][2], pv[3][3]);
+            double rhoL = um1[RHO], rhoR = u0[RHO];
+            double rhoeL = um1[RHOE] - kHalf * (um1[RHOU] * um1[RHOU] + um1[RHOV] * um1[RHOV] + um1[RHOW] * um1[RHOW]) / um1[RHO];
+            double rhoeR = u0[RHOE] - kHalf * (u0[RHOU] * u0[RHOU] + u0[RHOV] * u0[RHOV] + u0[RHOW] * u0[RHOW]) / u0[RHO];
+            double presL = P::get_p_from_rec(T, rhoeL, um1[RHO], um1 + NCV);
+            double presR = P::get_p_from_rec(T, rhoeR, u0[RHO], u0 + NCV);
+            double veln = fmax(fabs(um1[RHOU + D] / um1[RHO]), fabs(u0[RHOU + D] / u0[RHO]));
+            double gamaL = P::get_gama_from_rpc(T, rhoL, presL, um1 + NCV);
+            double gamaR = P::get_gama_from_rpc(T, rhoR, presR, u0 + NCV);
+            double ahalf = fmax(sqrt(gamaL * presL / rhoL), sqrt(gamaR * presR / rhoR));
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                double fm2 = N::template euler_comp<D>(c, um2, pv[0][0], pv[0][1], pv[0][2], pv[0][3]);
+                double fm1 = N::template euler_comp<D>(c, um1, pv[1][0], pv[1][1], pv[1][2], pv[1][3]);
+                double f0 = N::template euler_comp<D>(c, u0, pv[2][0], pv[2][1], pv[2][2], pv[2][3]);
+                double fp1 = N::template euler_comp<D>(c, up1, pv[3][0], pv[3][1], pv[3][2], pv[3][3]);
+                double v = kTwelveth * (7 * f0 + 7 * fm1 - fm2 - fp1);
+                v -= kTwelveth * (3 * u0[c] - 3 * um1[c] - up1[c] + u0[c]) * (veln + ahalf) * a.dissfactor;
+                fput(c, adv(c, v));
+            }
         }
     }
-    // species diffusion (:425-428)
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-        double D_s = N::harmonic_mean(dl[s], dr[s]);
-        double cr = acc.get(i, j, k, FLO + s, cR), cl = acc.get(iL, jL, kL, FLO + s, cL);
-        Fo[NCV + s] -= D_s * (cr - cl) / dxs;
-    }
-#pragma unroll
-    for (int c = 0; c < NC; ++c) F(i, j, k, flux_comp(c, a.flux_full)) = Fo[c];
-    if (a.flux_full) {
+    if (full) {
 #pragma unroll
         for (int n = NCV; n < FLO; ++n) F(i, j, k, n) = kZero;
     }
+
+    face_pass2<P, D>(T, acc, a, i, j, k, F, pL, pR, cL, cR, hyp);
 }
 
 template <class P, int ORDER, int D>
@@ -306,7 +346,8 @@ __global__ __launch_bounds__(256) void k_faces(StageArgs a, Work w) {
     if (step_skipped(a.ts)) return;
     const Tab &T = *a.tab;
     const double time = stage_time(a);
-    for (int t = w.t0 + blockIdx.x; t < w.t1; t += gridDim.x) {
+    for (TileIter it(w); it.ok(); it.next()) {
+        const int t = it.t;
         int lt;
         const BoxDesc &b = tile_box(w, t, lt);
         int c0[3], c1[3];
@@ -327,34 +368,184 @@ __global__ __launch_bounds__(256) void k_faces(StageArgs a, Work w) {
 }
 
 // ---------------------------------------------------------------------------
+// WENO5 face sweep of direction D with the stencil staged in LDS. A tile of
+// faces first stages every stencil cell once (conserved state, the velocities
+// and pressure of geteulerflux, gamma and vfrac); each face then reads its six
+// cells from LDS. Same arithmetic as face_flux<P,5,D>; the plugin's gamma must
+// depend on the species only (then gamma(rho_R, presR, spec_R) of the
+// eigenvalue equals the staged gamma of the right cell).
+// ---------------------------------------------------------------------------
+template <class P, int D>
+__global__ __launch_bounds__(256) void k_faces_lds(StageArgs a, Work w) {
+    using N = Num<P>;
+    constexpr int NS = P::NS, NC = N::NC;
+    constexpr int e0 = D == 0, e1 = D == 1, e2 = D == 2;
+    constexpr int FVX = NC, FP = NC + 3, FG = NC + 4, FVF = NC + 5;
+    static_assert(P::kGammaIgnoresRhoP, "staged gamma needs a species-only gamma");
+    extern __shared__ double lds[];
+    if (step_skipped(a.ts)) return;
+    const Tab &T = *a.tab;
+    const bool ns = a.nsflag != 0;
+    const double dxs = a.dx[D];
+    for (TileIter it(w); it.ok(); it.next()) {
+        const int t = it.t;
+        int lt;
+        const BoxDesc &b = tile_box(w, t, lt);
+        int c0[3], c1[3];
+        tile_range(b, w, lt, c0, c1);
+        if (c1[D] == b.hi[D]) c1[D] += 1;  // growHi(bx, D, 1)
+        const int fx = c1[0] - c0[0] + 1, fy = c1[1] - c0[1] + 1, fz = c1[2] - c0[2] + 1;
+        const int sx = fx + 5 * e0, sy = fy + 5 * e1, sz = fz + 5 * e2;
+        const int ncs = sx * sy * sz;
+        const int ox = c0[0] - 3 * e0, oy = c0[1] - 3 * e1, oz = c0[2] - 3 * e2;
+        Acc acc = make_acc(a, b);
+        if (ns) {
+            for (int q = threadIdx.x; q < ncs; q += blockDim.x) {
+                const int ix = q % sx, r = q / sx, iy = r % sy, iz = r / sy;
+                const int ci = ox + ix, cj = oy + iy, ck = oz + iz;
+                const CellPtr cp = cell_ptr(acc, ci, cj, ck, acc.cid(ci, cj, ck));
+                double u[NC];
+#pragma unroll
+                for (int n = 0; n < NCV; ++n) u[n] = cp(n);
+#pragma unroll
+                for (int n = 0; n < NS; ++n) u[NCV + n] = cp(FLO + n);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) lds[c * ncs + q] = u[c];
+                double vx, vy, vz, p;
+                N::euler_prims(T, u, vx, vy, vz, p);
+                lds[(FVX + 0) * ncs + q] = vx;
+                lds[(FVX + 1) * ncs + q] = vy;
+                lds[(FVX + 2) * ncs + q] = vz;
+                lds[FP * ncs + q] = p;
+                lds[FG * ncs + q] = P::get_gama_from_rpc(T, u[RHO], p, u + NCV);
+                lds[FVF * ncs + q] = cp(VFRAC);
+            }
+        }
+        __syncthreads();
+        const int nf = fx * fy * fz;
+        const int sD = D == 0 ? 1 : (D == 1 ? sx : sx * sy);
+        for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+            const int ix = f % fx, r = f / fx, iy = r % fy, iz = r / fy;
+            const int i = c0[0] + ix, j = c0[1] + iy, k = c0[2] + iz;
+            const int q0 = ix + sx * (iy + sy * iz);
+            auto Ld = [&](int field, int m) { return lds[field * ncs + q0 + m * sD]; };
+            const int iL = i - e0, jL = j - e1, kL = k - e2;
+            const int cL = acc.cid(iL, jL, kL), cR = acc.cid(i, j, k);
+            const CellPtr pL = cell_ptr(acc, iL, jL, kL, cL), pR = cell_ptr(acc, i, j, k, cR);
+            const View &F = b.flux[D];
+            const int full = a.flux_full;
+            double *Fp = F.p + F.off(i, j, k);
+            const int64_t Fs = F.ns;
+            auto fput = [&](int c, double v) { Fp[(int64_t)flux_comp(c, full) * Fs] = v; };
+            const double vfL = ns ? Ld(FVF, 2) : pL(VFRAC), vfR = ns ? Ld(FVF, 3) : pR(VFRAC);
+            const double facevfrac = kHalf * (vfL + vfR);
+            const bool hyp = ns && facevfrac >= 0.5;
+            if (!ns) {
+                double velface = kHalf * (pL(VELX + D) + pR(VELX + D));
+                if (a.spec_in_solid) {
+                    double wl = (vfL < kOne) ? kZero : kOne;
+                    velface = (wl * pL(VELX + D) + wl * pR(VELX + D)) / (wl + wl + kTiny);
+                }
+#pragma unroll
+                for (int c = 0; c < NCV; ++c) fput(c, kZero);
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    fput(NCV + s, T.advect_flag[s] * (kHalf * (velface + fabs(velface)) * pL(FLO + s) +
+                                                      kHalf * (velface - fabs(velface)) * pR(FLO + s)));
+            } else if (!hyp) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) fput(c, kZero);
+            } else {
+                bool first = false;
+#pragma unroll
+                for (int m = 0; m < 6; ++m) first = first || (Ld(FVF, m) < kOne);
+                const int m0 = first ? 2 : 0, m1 = first ? 2 : 1, m4 = first ? 3 : 4, m5 = first ? 3 : 5;
+                // max eigenvalue, flux_schemes.H:10-46 (presR with the left species)
+                double lam;
+                {
+                    double ul[NC], ur[NC];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) { ul[c] = Ld(c, 2); ur[c] = Ld(c, 3); }
+                    double rhoeL = ul[RHOE] - kHalf * (ul[RHOU] * ul[RHOU] + ul[RHOV] * ul[RHOV] + ul[RHOW] * ul[RHOW]) / ul[RHO];
+                    double rhoeR = ur[RHOE] - kHalf * (ur[RHOU] * ur[RHOU] + ur[RHOV] * ur[RHOV] + ur[RHOW] * ur[RHOW]) / ur[RHO];
+                    double presL = P::get_p_from_rec(T, rhoeL, ul[RHO], ul + NCV);
+                    double presR = P::get_p_from_rec(T, rhoeR, ur[RHO], ul + NCV);
+                    double veln = fmax(fabs(Ld(FVX + D, 2)), fabs(Ld(FVX + D, 3)));
+                    double gamaL = Ld(FG, 2), gamaR = Ld(FG, 3);
+                    double ahalf = fmax(sqrt(gamaL * presL / ul[RHO]), sqrt(gamaR * presR / ur[RHO]));
+                    lam = ahalf + veln;
+                }
+                const int ms[6] = {m0, m1, 2, 3, m4, m5};
+#pragma unroll 1
+                for (int c = 0; c < NC; ++c) {
+                    double uc[6], fc[6];
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        const int m = ms[q];
+                        uc[q] = Ld(c, m);
+                        const double vn = Ld(FVX + D, m);
+                        if (c == RHO) fc[q] = uc[q] * vn;
+                        else if (c >= RHOU && c <= RHOW) {
+                            const double va = Ld(FVX + (c - RHOU), m);
+                            const double mm = Ld(RHO, m) * va;
+                            fc[q] = (c - RHOU == D) ? (mm * va + Ld(FP, m)) : (mm * vn);
+                        } else if (c == RHOE) fc[q] = (uc[q] + Ld(FP, m)) * vn;
+                        else fc[q] = uc[q] * vn;
+                    }
+                    double fp = N::weno_plus(fc[0], fc[1], fc[2], fc[3], fc[4]);
+                    double fm = N::weno_minus(fc[1], fc[2], fc[3], fc[4], fc[5]);
+                    double up = N::weno_plus(uc[0], uc[1], uc[2], uc[3], uc[4]);
+                    double um = N::weno_minus(uc[1], uc[2], uc[3], uc[4], uc[5]);
+                    double h = kHalf * (fp + lam * up) + kHalf * (fm - lam * um);
+                    fput(c, c < NCV ? h : T.advect_flag[c - NCV] * h);
+                }
+            }
+            if (full) {
+#pragma unroll
+                for (int n = NCV; n < FLO; ++n) F(i, j, k, n) = kZero;
+            }
+            face_pass2<P, D>(T, acc, a, i, j, k, F, pL, pR, cL, cR, hyp);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Post-step work on one cell (fused epilogue of stage 3):
 // Advance_chemistry_implicit (SSPRK3 of AMReX's TimeIntegrator, rk.type=3),
 // update_primitive_vars, clip_neg_speciesconc, residual-norm and dt partials.
 // ---------------------------------------------------------------------------
+// stage state of the chemistry integrator as seen by the plugin: the species
+// of the stage, every other component of the post-flow state (unchanged by
+// the integrator: their right-hand side is zero)
+template <int NS>
+struct ChemCell {
+    const double *S;
+    const double *sp;
+    MFD double operator()(int n) const { return n >= FLO ? sp[n - FLO] : S[n]; }
+};
+
 template <class P>
 MFD void chemistry_rk3(const Tab &T, double *S, double dt) {
-    constexpr int NS = P::NS, NV = Num<P>::NV;
+    constexpr int NS = P::NS;
     if (!P::kHasChemistry) return;
-    double S0[NS], F[3][NS];
+    double S0[NS], F[3][NS], st[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) S0[s] = S[FLO + s];
     const double a10 = 1.0, a20 = 0.25, a21 = 0.25;
-    double cell[NV];
 #pragma unroll
-    for (int n = 0; n < NV; ++n) cell[n] = S[n];
-#pragma unroll
-    for (int st = 0; st < 3; ++st) {
+    for (int k = 0; k < 3; ++k) {
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             double v = S0[s];
-            if (st == 1) v += (dt * a10) * F[0][s];
-            if (st == 2) { v += (dt * a20) * F[0][s]; v += (dt * a21) * F[1][s]; }
-            cell[FLO + s] = v;
+            if (k == 1) v += (dt * a10) * F[0][s];
+            if (k == 2) { v += (dt * a20) * F[0][s]; v += (dt * a21) * F[1][s]; }
+            st[s] = v;
         }
         double src[NS];
-        P::spec_source(T, CellArr<NV>{cell}, src);
+        P::spec_source(T, ChemCell<NS>{S, st}, src);
 #pragma unroll
-        for (int s = 0; s < NS; ++s) F[st][s] = 0.0 + src[s];
+        for (int s = 0; s < NS; ++s) F[k][s] = 0.0 + src[s];
     }
     const double w0 = 1. / 6., w1 = 1. / 6., w2 = 2. / 3.;
 #pragma unroll
@@ -424,7 +615,10 @@ MFD void block_reduce_store(double *sums, double m0, double m1, double *out) {
 // update_residual (div_3d.H:13-76) and the RK3 stage arithmetic
 // (mflo_evolve.cpp:283, 296-298, 309-311) per cell of a tile.
 // ---------------------------------------------------------------------------
-template <class P>
+// EPI: stage-3 epilogue; CHEM: chemistry on (then the residual of the conserved
+// components and vfrac is exactly zero: S_new and S_old differ only in the
+// species and, after update_primitive_vars, the primitives).
+template <class P, bool EPI, bool CHEM>
 __global__ __launch_bounds__(256) void k_residual(ResArgs r, Work w) {
     using N = Num<P>;
     constexpr int NS = P::NS, NV = N::NV;
@@ -435,7 +629,8 @@ __global__ __launch_bounds__(256) void k_residual(ResArgs r, Work w) {
     const double dt = stage_dt(a);
     (void)time;
     const int nvn = (a.spec_in_solid == 1) ? FLO : FLO + NS;
-    for (int t = w.t0 + blockIdx.x; t < w.t1; t += gridDim.x) {
+    for (TileIter it(w); it.ok(); it.next()) {
+        const int t = it.t;
         int lt;
         const BoxDesc &b = tile_box(w, t, lt);
         int c0[3], c1[3];
@@ -502,8 +697,8 @@ __global__ __launch_bounds__(256) void k_residual(ResArgs r, Work w) {
                 }
             }
             if (!r.combine) continue;
-            if (r.epilogue) {
-                if (r.do_chem) {
+            if (EPI) {
+                if (CHEM) {
 #pragma unroll
                     for (int n = 0; n < NV; ++n) So[n] = S[n];
                     chemistry_rk3<P>(T, S, dt);
@@ -531,6 +726,7 @@ __global__ __launch_bounds__(256) void k_residual(ResArgs r, Work w) {
                     const double ia = 1.0 / dt, ib = -1.0 / dt;
 #pragma unroll
                     for (int n = 0; n < NV; ++n) {
+                        if (CHEM && (n < DENS || n == VFRAC)) continue;
                         double df = ia * S[n] + ib * So[n];
                         if (n < VFRAC) df *= So[VFRAC];
                         sums[n] += df * df;
@@ -544,7 +740,105 @@ __global__ __launch_bounds__(256) void k_residual(ResArgs r, Work w) {
 #pragma unroll
             for (int n = 0; n < NV; ++n) r.dst(i, j, k, n) = S[n];
         }
-        if (r.combine && r.epilogue) block_reduce_store<NV>(sums, mws, mD, r.partials + (size_t)t * (NV + 2));
+        if (EPI) block_reduce_store<NV>(sums, mws, mD, r.partials + (size_t)t * (NV + 2));
+    }
+}
+
+// Post-step pass over the level after the stage-3 combine: S3 is read from
+// st.src (A with chemistry, B without), U^n from `old` (needed by the residual
+// only without chemistry), the final state goes to `dst` (A):
+// Advance_chemistry_implicit, update_primitive_vars, clip_neg_speciesconc,
+// compute_residual_norms and EstTimeStep partials, point by point.
+template <class P, bool CHEM>
+__global__ __launch_bounds__(256) void k_post(ResArgs r, Work w) {
+    using N = Num<P>;
+    constexpr int NS = P::NS, NV = N::NV;
+    const StageArgs &a = r.st;
+    if (step_skipped(a.ts)) return;
+    const Tab &T = *a.tab;
+    const double dt = stage_dt(a);
+    for (TileIter it(w); it.ok(); it.next()) {
+        const int t = it.t;
+        int lt;
+        const BoxDesc &b = tile_box(w, t, lt);
+        int c0[3], c1[3];
+        tile_range(b, w, lt, c0, c1);
+        const int ex = c1[0] - c0[0] + 1, ey = c1[1] - c0[1] + 1, ez = c1[2] - c0[2] + 1;
+        const int nc = ex * ey * ez;
+        double sums[NV];
+#pragma unroll
+        for (int n = 0; n < NV; ++n) sums[n] = 0.0;
+        double mws = 0.0, mD = 0.0;
+        for (int f = threadIdx.x; f < nc; f += blockDim.x) {
+            const int i = c0[0] + f % ex;
+            const int rr = f / ex;
+            const int j = c0[1] + rr % ey;
+            const int k = c0[2] + rr / ey;
+            const double *srcp = a.src.p + a.src.off(i, j, k);
+            double *sp = r.dst.p + r.dst.off(i, j, k);
+            const int64_t sn = r.dst.ns;
+            double S[NV];
+#pragma unroll
+            for (int n = 0; n < NV; ++n) S[n] = srcp[(int64_t)n * a.src.ns];
+            // S_old of compute_residual_norms: the post-flow state with chemistry
+            // (second swap, mflo_evolve.cpp:473), else U^n
+            double So_p[NV - NCV];  // components DENS.. of S_old
+            if (CHEM) {
+#pragma unroll
+                for (int n = DENS; n < NV; ++n) So_p[n - DENS] = S[n];
+                chemistry_rk3<P>(T, S, dt);
+            }
+            // update_primitive_vars (mflo_evolve.cpp:129-160)
+            if (S[VFRAC] == kOne) {
+                double u[N::NC], p[NCV];
+#pragma unroll
+                for (int c = 0; c < NCV; ++c) u[c] = S[c];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) u[NCV + s] = S[FLO + s];
+                N::cons_to_prim(T, u, p);
+#pragma unroll
+                for (int c = 0; c < NCV; ++c) S[DENS + c] = p[c];
+                S[TEMP] = P::get_t_from_rpc(T, S[DENS], S[PRES], u + NCV);
+            } else if (a.conj_ht && S[VFRAC] == kZero) {
+                S[TEMP] = P::get_solid_t_from_rhoe(T, S[RHOE]);
+            }
+            if (r.do_ns && S[PRES] < 0) raise_err(a.err, MF_ERR_NEG_PRESSURE);
+            if (r.clip) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s) if (S[FLO + s] < 0.0) S[FLO + s] = 0.0;
+            }
+            if (r.track_res) {
+                const double ia = 1.0 / dt, ib = -1.0 / dt;
+                if (CHEM) {
+                    // conserved components and vfrac: S_new == S_old, exactly zero
+                    const double vfo = S[VFRAC];
+#pragma unroll
+                    for (int n = DENS; n < NV; ++n) {
+                        if (n == VFRAC) continue;
+                        double df = ia * S[n] + ib * So_p[n - DENS];
+                        if (n < VFRAC) df *= vfo;
+                        sums[n] += df * df;
+                    }
+                } else {
+                    const View &o = r.old;
+                    const double *op = o.p + o.off(i, j, k);
+                    const double vfo = op[(int64_t)VFRAC * o.ns];
+#pragma unroll
+                    for (int n = 0; n < NV; ++n) {
+                        double df = ia * S[n] + ib * op[(int64_t)n * o.ns];
+                        if (n < VFRAC) df *= vfo;
+                        sums[n] += df * df;
+                    }
+                }
+            }
+            double ws, md;
+            dt_partial<P>(T, S, r.do_ns, ws, md);
+            mws = fmax(mws, ws);
+            mD = fmax(mD, md);
+#pragma unroll
+            for (int n = 0; n < NV; ++n) sp[(int64_t)n * sn] = S[n];
+        }
+        block_reduce_store<NV>(sums, mws, mD, r.partials + (size_t)t * (NV + 2));
     }
 }
 
@@ -554,7 +848,8 @@ __global__ __launch_bounds__(256) void k_est(ResArgs r, Work w) {
     constexpr int NV = Num<P>::NV;
     const StageArgs &a = r.st;
     const Tab &T = *a.tab;
-    for (int t = w.t0 + blockIdx.x; t < w.t1; t += gridDim.x) {
+    for (TileIter it(w); it.ok(); it.next()) {
+        const int t = it.t;
         int lt;
         const BoxDesc &b = tile_box(w, t, lt);
         int c0[3], c1[3];
@@ -749,26 +1044,37 @@ MFD void cutcell_values(const Tab &T, const CutArgs &a, int i, int j, int k, dou
         for (int q3 = 0; q3 < 3; ++q3) out[VELX + q3] = (su[q3] + 0.0 / (dn * dn)) / (sw + wd);
     }
     // homogeneous Neumann IDW of p, T, rho (and species without spec_in_solid), :154-198
-    constexpr int NN = 3 + NS;
-    int comps[NN];
-    int nn = 0;
-    comps[nn++] = PRES;
-    if (!a.conj_ht) comps[nn++] = TEMP;
-    comps[nn++] = DENS;
-    if (!a.spec_in_solid) for (int sp = 0; sp < NS; ++sp) comps[nn++] = FLO + sp;
-    double sv[NN], sw = 0.0;
-    for (int c = 0; c < NN; ++c) sv[c] = 0.0;
-    for (int n = -2; n <= 2; n++)
-        for (int m = -2; m <= 2; m++)
-            for (int l = -2; l <= 2; l++) {
-                if (s(i + l, j + m, k + n, VFRAC) == 1.0) {
-                    double a0 = l * dnd[0] - xnd[0], a1 = m * dnd[1] - xnd[1], a2 = n * dnd[2] - xnd[2];
-                    double w2 = a0 * a0 + a1 * a1 + a2 * a2 + kTiny2;
-                    for (int c = 0; c < nn; ++c) sv[c] += s(i + l, j + m, k + n, comps[c]) / w2;
-                    sw += 1.0 / w2;
+    {
+        const bool doT = !a.conj_ht, doS = !a.spec_in_solid;
+        double sp_ = 0.0, sT = 0.0, sR = 0.0, sw = 0.0;
+        double sS[NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) sS[q] = 0.0;
+        for (int n = -2; n <= 2; n++)
+            for (int m = -2; m <= 2; m++)
+                for (int l = -2; l <= 2; l++) {
+                    if (s(i + l, j + m, k + n, VFRAC) == 1.0) {
+                        double a0 = l * dnd[0] - xnd[0], a1 = m * dnd[1] - xnd[1], a2 = n * dnd[2] - xnd[2];
+                        double w2 = a0 * a0 + a1 * a1 + a2 * a2 + kTiny2;
+                        sp_ += s(i + l, j + m, k + n, PRES) / w2;
+                        if (doT) sT += s(i + l, j + m, k + n, TEMP) / w2;
+                        sR += s(i + l, j + m, k + n, DENS) / w2;
+                        if (doS) {
+#pragma unroll
+                            for (int q = 0; q < NS; ++q) sS[q] += s(i + l, j + m, k + n, FLO + q) / w2;
+                        }
+                        sw += 1.0 / w2;
+                    }
                 }
-            }
-    for (int c = 0; c < nn; ++c) out[comps[c]] = (sw > 0.0) ? sv[c] / sw : s(i, j, k, comps[c]);
+        const bool any = sw > 0.0;
+        out[PRES] = any ? sp_ / sw : s(i, j, k, PRES);
+        if (doT) out[TEMP] = any ? sT / sw : s(i, j, k, TEMP);
+        out[DENS] = any ? sR / sw : s(i, j, k, DENS);
+        if (doS) {
+#pragma unroll
+            for (int q = 0; q < NS; ++q) out[FLO + q] = any ? sS[q] / sw : s(i, j, k, FLO + q);
+        }
+    }
     if (a.conj_ht) {
         double st = 0.0;
         int cnt = 0;
@@ -856,8 +1162,34 @@ struct Launch {
         default: faces_o<5>(a, w, dir, grid, block, st); break;
         }
     }
+    static size_t lds_bytes(const int *tsh, int dir) {
+        int e[3] = {tsh[0], tsh[1], tsh[2]};
+        e[dir] += 6;
+        return (size_t)e[0] * e[1] * e[2] * (Num<P>::NC + 6) * sizeof(double);
+    }
+    static void faces_lds(const StageArgs &a, const Work &w, int dir, int grid, int block, size_t lds, hipStream_t st) {
+        static bool attr = false;
+        if (!attr) {
+            hipFuncSetAttribute((const void *)k_faces_lds<P, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            hipFuncSetAttribute((const void *)k_faces_lds<P, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            hipFuncSetAttribute((const void *)k_faces_lds<P, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            attr = true;
+        }
+        if (dir == 0) hipLaunchKernelGGL((k_faces_lds<P, 0>), dim3(grid), dim3(block), lds, st, a, w);
+        else if (dir == 1) hipLaunchKernelGGL((k_faces_lds<P, 1>), dim3(grid), dim3(block), lds, st, a, w);
+        else hipLaunchKernelGGL((k_faces_lds<P, 2>), dim3(grid), dim3(block), lds, st, a, w);
+    }
     static void residual(const ResArgs &r, const Work &w, int grid, int block, hipStream_t st) {
-        hipLaunchKernelGGL(k_residual<P>, dim3(grid), dim3(block), 0, st, r, w);
+        if (r.combine && r.epilogue) {
+            if (r.do_chem) hipLaunchKernelGGL((k_residual<P, true, true>), dim3(grid), dim3(block), 0, st, r, w);
+            else hipLaunchKernelGGL((k_residual<P, true, false>), dim3(grid), dim3(block), 0, st, r, w);
+        } else {
+            hipLaunchKernelGGL((k_residual<P, false, false>), dim3(grid), dim3(block), 0, st, r, w);
+        }
+    }
+    static void post(const ResArgs &r, const Work &w, int grid, int block, hipStream_t st) {
+        if (r.do_chem) hipLaunchKernelGGL((k_post<P, true>), dim3(grid), dim3(block), 0, st, r, w);
+        else hipLaunchKernelGGL((k_post<P, false>), dim3(grid), dim3(block), 0, st, r, w);
     }
     static void est(const ResArgs &r, const Work &w, int grid, int block, hipStream_t st) {
         hipLaunchKernelGGL(k_est<P>, dim3(grid), dim3(block), 0, st, r, w);
@@ -869,8 +1201,11 @@ struct Launch {
         k.cut = &cut;
         k.cut_box = &cut_box;
         k.faces = &faces;
+        k.faces_lds = &faces_lds;
+        k.lds_bytes = &lds_bytes;
         k.residual = &residual;
         k.est = &est;
+        k.post = &post;
         return k;
     }
 };

@@ -37,10 +37,11 @@ SIGNATURES = {
     "mf_fill_ghosts_single_level": ([C.c_void_p, C.c_double], C.c_int),
     "mf_est_dt": ([C.c_void_p, _P(C.c_double)], C.c_int),
     "mf_level_step": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "mf_level_step_timed": ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
     "mf_level_info": ([C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_int), C.c_void_p], C.c_int),
     "mf_level_set_time": ([C.c_void_p, C.c_double, C.c_double, C.c_int], C.c_int),
-    "mf_level_set_tile": ([C.c_void_p, C.c_int, C.c_int, C.c_int], C.c_int),
-    "mf_level_get_tile": ([C.c_void_p, _P(C.c_int), _P(C.c_int), _P(C.c_int)], C.c_int),
+    "mf_level_set_tile": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+    "mf_level_get_tile": ([C.c_void_p, C.c_int, _P(C.c_int), _P(C.c_int), _P(C.c_int)], C.c_int),
     "mf_level_autotune": ([C.c_void_p], C.c_int),
     "mf_adjoint_forward": ([C.c_void_p, C.c_int, C.c_int, C.c_int, _P(C.c_double)], C.c_int),
     "mf_adjoint_backward": ([C.c_void_p, C.c_void_p, _P(C.c_double)], C.c_int),
@@ -156,6 +157,16 @@ class Level:
     def step(self, n=1, mode=abi.MODE_BATCHED):
         check(lib().mf_level_step(self.h, int(n), int(mode)), "mf_level_step")
 
+    KERNEL_CLASSES = ("compute_dt", "fill_ghosts", "cutcells", "face_flux", "residual_combine", "finalize", "post_step")
+
+    def step_timed(self, n=1, mode=abi.MODE_BATCHED):
+        """Steps with HIP events around each kernel class; returns {class: (ms, launches)}."""
+        ms = np.zeros(7)
+        cnt = np.zeros(7, dtype=np.int32)
+        check(lib().mf_level_step_timed(self.h, int(n), int(mode), ms.ctypes.data_as(C.c_void_p),
+                                        cnt.ctypes.data_as(C.c_void_p)), "mf_level_step_timed")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(self.KERNEL_CLASSES)}
+
     def est_dt(self):
         d = C.c_double()
         check(lib().mf_est_dt(self.h, C.byref(d)), "mf_est_dt")
@@ -188,17 +199,26 @@ class Level:
     def set_time(self, t, dt_prev, istep):
         check(lib().mf_level_set_time(self.h, float(t), float(dt_prev), int(istep)), "mf_level_set_time")
 
-    def set_tile(self, tx, ty, tz):
-        check(lib().mf_level_set_tile(self.h, int(tx), int(ty), int(tz)), "mf_level_set_tile")
+    TILE_CLASSES = ("faces_x", "faces_y", "faces_z", "cells")
 
-    def get_tile(self):
+    def set_tile(self, cls, tx, ty, tz):
+        check(lib().mf_level_set_tile(self.h, int(cls), int(tx), int(ty), int(tz)), "mf_level_set_tile")
+
+    def get_tile(self, cls):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
-        check(lib().mf_level_get_tile(self.h, C.byref(a), C.byref(b), C.byref(c)), "mf_level_get_tile")
+        check(lib().mf_level_get_tile(self.h, int(cls), C.byref(a), C.byref(b), C.byref(c)), "mf_level_get_tile")
         return a.value, b.value, c.value
+
+    def tiles(self):
+        return [self.get_tile(q) for q in range(4)]
+
+    def set_tiles(self, tiles):
+        for q, t in enumerate(tiles):
+            self.set_tile(q, *t)
 
     def autotune(self):
         check(lib().mf_level_autotune(self.h), "mf_level_autotune")
-        return self.get_tile()
+        return self.tiles()
 
 
 _HIPMEMCPY = None

@@ -974,12 +974,14 @@ struct Level : LevelBase {
         A4 S = valid(const_cast<std::vector<double> &>(src));
         A4 F = padded(out);
         int lo[3] = {0, 0, 0}, hi[3] = {n[0] - 1, n[1] - 1, n[2] - 1};
+#pragma omp parallel for collapse(2) schedule(static)
         for (int c = 0; c < NV; ++c)
             for (int k = 0; k < n[2]; ++k)
                 for (int j = 0; j < n[1]; ++j)
                     for (int i = 0; i < n[0]; ++i) F(i, j, k, c) = S(i, j, k, c);
         const int *per = geom.is_periodic;
         // phase 0: periodic images (cells outside only in periodic directions)
+#pragma omp parallel for schedule(static)
         for (int k = -3; k < n[2] + 3; ++k)
             for (int j = -3; j < n[1] + 3; ++j)
                 for (int i = -3; i < n[0] + 3; ++i) {
@@ -998,6 +1000,7 @@ struct Level : LevelBase {
                     }
                 }
         for (int phase = 1; phase <= 3; ++phase) {
+#pragma omp parallel for schedule(static)
             for (int k = -3; k < n[2] + 3; ++k)
                 for (int j = -3; j < n[1] + 3; ++j)
                     for (int i = -3; i < n[0] + 3; ++i) {
@@ -1066,6 +1069,7 @@ struct Level : LevelBase {
             for (int d = 0; d < 3; ++d) { glo[d] = b.lo[d] - 3; ghi[d] = b.hi[d] + 3; }
             std::vector<double> sbv((size_t)(ghi[0] - glo[0] + 1) * (ghi[1] - glo[1] + 1) * (ghi[2] - glo[2] + 1) * NV);
             A4 sb = make_a4(sbv.data(), glo, ghi, NV);
+#pragma omp parallel for collapse(2) schedule(static)
             for (int c = 0; c < NV; ++c)
                 for (int k = glo[2]; k <= ghi[2]; ++k)
                     for (int j = glo[1]; j <= ghi[1]; ++j)
@@ -1073,6 +1077,7 @@ struct Level : LevelBase {
             box_cutcells(pl, b, sb, dx, prm.species_in_solid, prm.conj_ht);
             int st = box_rhs(pl, fl, b, sb, D, nullptr, dx, rhs_time);
             if (st != MF_OK) return st;
+#pragma omp parallel for collapse(2) schedule(static)
             for (int c = 0; c < NV; ++c)
                 for (int k = b.lo[2]; k <= b.hi[2]; ++k)
                     for (int j = b.lo[1]; j <= b.hi[1]; ++j)
@@ -1086,6 +1091,7 @@ struct Level : LevelBase {
         A4 S = valid(phi_new);
         double max_ws = 0.0, max_D = 0.0;
         double do_ns_flag = prm.do_ns;
+#pragma omp parallel for schedule(static) reduction(max : max_ws, max_D)
         for (int k = 0; k < n[2]; ++k)
             for (int j = 0; j < n[1]; ++j)
                 for (int i = 0; i < n[0]; ++i) {
@@ -1141,10 +1147,12 @@ struct Level : LevelBase {
         // stage 1
         int st = dsdt_flow(phi_old, time, time, sbv);
         if (st) return st;
+#pragma omp parallel for schedule(static)
         for (long q = 0; q < N; ++q) phi_new[q] = one * sbv[q] + dt_lev * dsdt[q];  // LinComb
         // stage 2
         st = dsdt_flow(phi_new, time + dt_lev, time + dt_lev, sbv);
         if (st) return st;
+#pragma omp parallel for schedule(static)
         for (long q = 0; q < N; ++q) {
             double v = phi_new[q];
             v += dt_lev * dsdt[q];        // Saxpy
@@ -1154,6 +1162,7 @@ struct Level : LevelBase {
         // stage 3
         st = dsdt_flow(phi_new, time + dt_lev, time + dt_lev, sbv);
         if (st) return st;
+#pragma omp parallel for schedule(static)
         for (long q = 0; q < N; ++q) {
             double v = phi_new[q];
             v += dt_lev * dsdt[q];        // Saxpy
@@ -1203,6 +1212,7 @@ struct Level : LevelBase {
         const long nc = ncell();
         int bad = 0;
         double minp_all = std::numeric_limits<double>::max();
+#pragma omp parallel for schedule(static)
         for (long q = 0; q < nc; ++q) {
             double *s = phi_new.data();
             if (s[(size_t)VFRAC * nc + q] == one) {
@@ -1216,12 +1226,14 @@ struct Level : LevelBase {
                 s[(size_t)TEMP * nc + q] = pl.get_solid_t_from_rhoe(s[(size_t)RHOE * nc + q]);
             }
         }
+#pragma omp parallel for schedule(static) reduction(min : minp_all)
         for (long q = 0; q < nc; ++q) minp_all = std::min(minp_all, phi_new[(size_t)PRES * nc + q]);
         if (minp_all < 0 && prm.do_ns) bad = 1;
         return bad ? MF_ERR_NEG_PRESSURE : MF_OK;
     }
     void clip_neg_speciesconc() {  // Source/mflo_evolve.cpp:521-540
         const long nc = ncell();
+#pragma omp parallel for collapse(2) schedule(static)
         for (int sp = 0; sp < NS; sp++)
             for (long q = 0; q < nc; ++q) {
                 double &v = phi_new[(size_t)(FLO_NVARS + sp) * nc + q];
@@ -1232,6 +1244,7 @@ struct Level : LevelBase {
         const long nc = ncell();
         resnorms.assign(NV, 0.0);
         double a = 1.0 / dt, b = -1.0 / dt;
+#pragma omp parallel for schedule(static)
         for (int c = 0; c < NV; ++c) {
             double sum = 0.0;
             for (long q = 0; q < nc; ++q) {
