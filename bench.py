@@ -4,7 +4,7 @@ Workload (default): single-level 256^3 FP64 reacting flow through a porous solid
 seeded packed spheres (radii U[6,14] cells, porosity 0.6) as the immersed boundary,
 5 species (4 gas + 1 site), 2 Arrhenius reactions, inflow u = 1 m/s at T = 700 K,
 WENO5 fluxes (order_hyp = 5), SSP-RK3, viscous/heat/species diffusion, explicit
-chemistry, CFL 0.5. A "step" is one RK3 step of the whole level with inputs
+chemistry, CFL 0.3. A "step" is one RK3 step of the whole level with inputs
 resident in HBM.
 
 Prints one JSON line: cell-updates/s (value), the roofline of the dominant kernel
