@@ -34,10 +34,18 @@ def log(*a):
 def make_case(name):
     from mesoflow_amd import cases
     if name == "porous256":
+        # 32^3 boxes: amr.max_grid_size of the reference's porous-particle model
+        # (models/porousParticle/inputs:17)
         c = cases.porous_reactor(n_cell=(256, 256, 256), nspec=5, nreact=2, geometry="spheres", porosity=0.6,
-                                 rmin=6.0, rmax=14.0, order_hyp=5, cfl=0.3, seed=20261203)
+                                 rmin=6.0, rmax=14.0, order_hyp=5, cfl=0.3, seed=20261203, max_grid_size=32)
         desc = ("porous reactor 256^3 FP64, 5 species, 2 Arrhenius reactions, IB sphere pack r~U[6,14] "
-                "porosity 0.6, inflow u=1 m/s T=700 K, WENO5 (order_hyp=5), single level, SSP-RK3")
+                "porosity 0.6, inflow u=1 m/s T=700 K, WENO5 (order_hyp=5), single level of 512 boxes of 32^3, SSP-RK3")
+    elif name == "uneven256":
+        boxes = cases.random_boxes((256, 256, 256), 8, 64, seed=7, p_stop=0.5)
+        c = cases.porous_reactor(n_cell=(256, 256, 256), nspec=5, nreact=2, geometry="grf", porosity=0.5,
+                                 corr_len=8.0, order_hyp=5, cfl=0.3, seed=20261205, boxes=boxes)
+        desc = ("porous reactor 256^3 FP64, 5 species, 2 reactions, GRF IB porosity 0.5, WENO5, single level "
+                "of %d uneven boxes with edges in [8,64], SSP-RK3" % len(boxes))
     elif name == "sod64":
         c = cases.shocktube_n2(n_cell=(64, 64, 64), prob_hi=(1.0, 1.0, 1.0), order_hyp=5, cfl=0.5)
         desc = "Sod shock tube 64^3 FP64, 1 species, WENO5, SSP-RK3 (CPU-runnable reference case)"
@@ -107,6 +115,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile", default=None, help="tile shapes of the 4 kernel classes 'x,y,z;x,y,z;x,y,z;x,y,z' (skips the autotuner)")
     ap.add_argument("--calibrate", action="store_true", help="plain copy kernel for PMC scale factors")
+    ap.add_argument("--compare-steps", type=int, default=3,
+                    help="steps timed in each launch mode after the main region (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -187,6 +197,22 @@ def main():
         extra["kernel_time_share"] = {k: round(v[0] / tot, 4) for k, v in classes.items()}
     extra["step_hbm_frac"] = (step_bytes * args.steps / secs / 1e9) / HBM_PEAK_GBS
     extra["tile"] = ";".join(",".join(str(v) for v in t) for t in lvl.tiles())
+
+    if args.compare_steps > 0:
+        # the same steps launched per box (the reference's shape), as persistent
+        # launches over the tile list, and replayed from a hipGraph; no events
+        modes = {}
+        for name, m in (("per_box", abi.MODE_PER_BOX), ("batched", abi.MODE_BATCHED), ("graph", abi.MODE_GRAPH)):
+            lvl.step(1, m)
+            torch.cuda.synchronize()
+            ta = time.perf_counter()
+            lvl.step(args.compare_steps, m)
+            torch.cuda.synchronize()
+            modes[name] = (time.perf_counter() - ta) / args.compare_steps * 1e3
+            log("mode %s: %.3f ms/step" % (name, modes[name]))
+        extra["ms_per_step_by_mode"] = modes
+        extra["batched_speedup_vs_per_box"] = modes["per_box"] / modes["batched"]
+        extra["graph_speedup_vs_per_box"] = modes["per_box"] / modes["graph"]
 
     cpu = None
     if not args.no_cpu_baseline:

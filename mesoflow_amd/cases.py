@@ -108,9 +108,10 @@ def chop_boxes(n_cell, max_grid_size):
     return boxes
 
 
-def random_boxes(n_cell, edge_lo, edge_hi, seed):
+def random_boxes(n_cell, edge_lo, edge_hi, seed, p_stop=0.35):
     """Uneven box decomposition: recursive bisection at random positions until
-    every edge is within [edge_lo, edge_hi] (boxes still tile the domain)."""
+    every edge is within [edge_lo, edge_hi] (boxes still tile the domain); a box
+    within the limits stops splitting with probability p_stop."""
     rng = np.random.default_rng(seed)
     out = []
     stack = [((0, 0, 0), tuple(n - 1 for n in n_cell))]
@@ -119,7 +120,7 @@ def random_boxes(n_cell, edge_lo, edge_hi, seed):
         ext = [hi[d] - lo[d] + 1 for d in range(3)]
         big = [d for d in range(3) if ext[d] > edge_hi]
         cand = [d for d in range(3) if ext[d] >= 2 * edge_lo]
-        if not big and (not cand or rng.random() < 0.35):
+        if not big and (not cand or rng.random() < p_stop):
             out.append((lo, hi))
             continue
         d = big[int(rng.integers(len(big)))] if big else cand[int(rng.integers(len(cand)))]
