@@ -151,6 +151,12 @@ int mf_device_count(int *count);
 int mf_ctx_create(const mf_plugin_params *plugin, const mf_params *params,
                   const mf_geom *geom, void **ctx_out);
 int mf_ctx_destroy(void *ctx);
+/* Replaces the plugin parameter table of a context (same kind, nspec, ngas;
+ * MF_ERR_BAD_ARG otherwise), e.g. new rate constants or inflow state. Work
+ * enqueued afterwards uses the new table. No counterpart in the reference
+ * (its tables are compile-time globals); used by parameter sweeps and the
+ * finite-difference checks of the adjoint. */
+int mf_ctx_set_plugin_params(void *ctx, const mf_plugin_params *plugin);
 /* All later work of the context is enqueued on this stream (NULL = default). */
 int mf_ctx_set_stream(void *ctx, void *hip_stream);
 int mf_ctx_nvars(void *ctx, int *nv);          /* TOTAL_NVARS = 12 + NUM_SPECIES */
@@ -239,12 +245,22 @@ int mf_level_autotune(void *level);
 /* ------------------------------------------------------------------------ */
 /* Discrete adjoint (no counterpart in the reference)                         */
 /* ------------------------------------------------------------------------ */
-/* Objective J = sum_n dt * (outlet species mass flux through x-hi of species
- * `spec`), accumulated over nsteps fixed-dt steps. Forward stores binomial
- * checkpoints (ncheck states); backward returns dJ/d(log_A[r]) for r < nreact
- * and dJ/d(in_vel). */
+/* Objective J = sum over the nsteps steps of dt * dy * dz * sum_{j,k} of
+ * molwt[spec] * c_spec * (rho u) / rho in the outlet cell layer i = nx-1 of
+ * the state after the step (the species mass flux leaving through x-hi).
+ * mf_adjoint_forward runs nsteps steps of dt_fixed from the current state and
+ * time, returns J and keeps binomial checkpoints (ncheck state snapshots,
+ * 1 <= ncheck; ncheck >= nsteps stores every step). mf_adjoint_backward then
+ * reverses the run with the discrete adjoint kernels (recomputing forward
+ * segments from the checkpoints) and returns dJ/d(log_A[r]) for r < nreact and
+ * dJ/d(in_vel); afterwards the level holds the initial state of the run again.
+ * Supported: the Arrhenius porous-reactor families with fixed_dt = 1,
+ * do_ns = 1, order_hyp = 5, conj_ht = 0, do_chemistry = 1
+ * (MF_ERR_UNSUPPORTED otherwise). */
 int mf_adjoint_forward(void *level, int nsteps, int ncheck, int spec, double *J_out);
 int mf_adjoint_backward(void *level, double *grad_logA, double *grad_invel);
+/* dJ/d(initial state) of the last backward sweep, [nv][nz][ny][nx]. */
+int mf_adjoint_state_grad(void *level, double *out, int is_device);
 
 /* Plain HBM copy of n doubles on the context stream (bandwidth calibration). */
 int mf_util_copy(void *ctx, double *dst, const double *src, int64_t n);

@@ -23,9 +23,11 @@ struct TimeState {
 };
 
 // Tile classes: 0..2 the LDS-staged face sweeps of direction 0..2, 3 cells
-// (residual/combine, dt estimate and the face sweeps without LDS staging).
-constexpr int kTileClasses = 4;
+// (residual/combine, dt estimate and the face sweeps without LDS staging),
+// 4..6 the adjoint face sweeps of direction 0..2.
+constexpr int kTileClasses = 7;
 constexpr int kCellClass = 3;
+constexpr int kAdjFaceClass = 4;
 
 // One box of a level (or the single box of a per-box call).
 struct BoxDesc {
@@ -102,6 +104,53 @@ struct CutArgs {
     int *flag;
 };
 
+// ---------------------------------------------------------------------------
+// discrete adjoint
+// ---------------------------------------------------------------------------
+struct AdjFaceArgs {
+    StageArgs st;              // forward stage: input array, its cut values, dx, flags
+    View lam_out;              // adjoint of the stage output (valid cells)
+    double coef;               // d(stage output)/d(dsdt): dt, dt/4 or 2 dt third
+    View lam;                  // adjoint of the stage input array (padded, accumulated)
+    double *lcut;              // adjoint of the stage's cut-cell values [nv][ncut] (accumulated)
+};
+struct AdjCutArgs {
+    CutArgs c;                 // forward stage input array and cut list
+    View lam;                  // adjoint of the input array (accumulated)
+    const double *lcut;        // adjoint of the cut-cell values
+};
+struct AdjFillArgs {
+    FillArgs f;                // forward array (its interior values feed externalbc)
+    View lam;                  // adjoint of the array: ghosts are folded into their sources and cleared
+    double *grad;              // grad[nreact] += d/d(in_vel)
+};
+struct AdjPostArgs {
+    StageArgs st;              // src = S3 (post-flow state); ts gives dt
+    View lu;                   // adjoint of U^{n+1}
+    View ld;                   // out: adjoint of S3
+    int clip;
+    int spec;                  // objective species
+    double dA;                 // outlet face area of a cell
+    int outlet_i;              // i of the outlet cell layer
+    double *grad;              // grad[r] += d/d(log_A[r])
+};
+struct ObjArgs {
+    const Tab *tab;
+    View s;                    // post-step state
+    const TimeState *ts;
+    int spec, outlet_i, ny, nz;
+    double dA;
+    double *J;                 // J += dt * outlet flux
+};
+struct AdjSet {
+    void (*faces)(const AdjFaceArgs &, const Work &, int dir, int grid, int block, size_t lds, hipStream_t);
+    size_t (*lds_bytes)(const int *tsh, int dir);
+    void (*cut)(const AdjCutArgs &, hipStream_t);
+    void (*fill)(const AdjFillArgs &, int phase, hipStream_t);
+    void (*post)(const AdjPostArgs &, const Work &, int grid, int block, hipStream_t);
+    void (*objective)(const ObjArgs &, hipStream_t);
+};
+
 // Host launchers of one compiled plugin family.
 struct KernelSet {
     int ns;
@@ -115,6 +164,7 @@ struct KernelSet {
     void (*residual)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
     void (*est)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
     void (*post)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
+    const AdjSet *adj;         // discrete adjoint (null: not compiled for this family)
 };
 
 extern const KernelSet ks_shocktube_n2;

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <new>
 #include <vector>
 #include "mf_args.h"
@@ -75,8 +76,8 @@ struct Level {
     std::vector<mf_box> boxes;
     std::vector<BoxDesc> hbox;
     BoxDesc *d_box = nullptr;
-    int ntiles[kTileClasses] = {0, 0, 0, 0};
-    int tsh[kTileClasses][3] = {{64, 4, 1}, {32, 8, 1}, {32, 1, 8}, {64, 2, 2}};
+    int ntiles[kTileClasses] = {0, 0, 0, 0, 0, 0, 0};
+    int tsh[kTileClasses][3] = {{64, 4, 1}, {32, 8, 1}, {32, 1, 8}, {64, 2, 2}, {16, 4, 2}, {16, 4, 2}, {16, 2, 4}};
     int max_grid = 2048;
     double *arr[3] = {nullptr, nullptr, nullptr};  // A, B, C
     size_t padded = 0;
@@ -92,6 +93,25 @@ struct Level {
     bool need_est = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    struct AdjState *adj = nullptr;
+};
+
+// Discrete adjoint of a fixed-dt run of nsteps steps (mf_adjoint_forward /
+// mf_adjoint_backward): binomial checkpoints of the level state, the stage
+// arrays of the step being reversed and the adjoint arrays.
+struct AdjState {
+    int nsteps = 0, ncheck = 0, spec = 0, ncut = -1;
+    bool ready = false;
+    double t0 = 0.0, dt = 0.0;
+    double *D = nullptr;                         // S3 of the recomputed step
+    double *lam[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // U^{n+1}, A (U^n), B, C, D
+    double *cutv[3] = {nullptr, nullptr, nullptr};                   // cut-cell values per stage
+    double *lcut[3] = {nullptr, nullptr, nullptr};                   // their adjoints
+    double *grad = nullptr;                      // [MF_MAX_REACTIONS + 1]
+    double *J = nullptr;
+    std::vector<double *> pool;                  // free snapshot buffers
+    std::vector<double *> bufs;                  // all snapshot buffers
+    std::map<int, double *> snaps;               // step -> snapshot of U^step
 };
 
 const KernelSet *pick_kernels(const mf_plugin_params &t) {
@@ -351,7 +371,7 @@ int rebuild_tiles(Level *L) {
     for (int q = 0; q < kTileClasses; ++q)
         for (int d = 0; d < 3; ++d) L->tsh[q][d] = std::max(1, L->tsh[q][d]);
     const int NC = NCV + c->ns;
-    int tiles[kTileClasses] = {0, 0, 0, 0};
+    int tiles[kTileClasses] = {0, 0, 0, 0, 0, 0, 0};
     size_t fofs = 0;
     L->hbox.resize(L->boxes.size());
     for (size_t b = 0; b < L->boxes.size(); ++b) {
@@ -558,6 +578,294 @@ void enqueue_est(Level *L) {
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, L->partials, L->ntiles[kCellClass], c->nv, L->ts, c->d_err, 1, 0);
 }
 
+// ---------------------------------------------------------------------------
+// discrete adjoint: host side
+// ---------------------------------------------------------------------------
+__global__ void k_set_ts(TimeState *ts, double t, double dt) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    ts->t = t;
+    ts->dt = dt;
+    ts->dt_prev = dt;
+    ts->active = 1;
+    ts->stop_time = 1.e300;
+}
+
+__global__ void k_zero(double *p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// Adjoint of the RK3 stage arithmetic over the valid cells:
+//   stage 3: S3 = (U^n + 2 (S2 + dt d)) third      -> la = third lo, lx = 2 third lo
+//   stage 2: S2 = ((S1 + dt d) + 3 U^n) / 4        -> lx = lo / 4, la += 3/4 lo
+//   stage 1: S1 = Sb + dt d (Sb: cut values at cut cells) -> lcut = lo (cut cells), la += lo
+// (the dsdt part goes through k_faces_adj)
+__global__ void k_adj_comb(int stage, View lo, View la, View lx, const int *cutid, double *lcut, int ncut, int nx, int ny,
+                           int nz, int nv) {
+    const int64_t ncell = (int64_t)nx * ny * nz;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ncell; q += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((int64_t)nx * ny));
+        const int cid = stage == 1 ? cutid[q] : -1;
+        for (int n = 0; n < nv; ++n) {
+            const double v = lo(i, j, k, n);
+            if (stage == 3) {
+                la(i, j, k, n) = kThird * v;
+                lx(i, j, k, n) = kTwo * kThird * v;
+            } else if (stage == 2) {
+                lx(i, j, k, n) = kFourth * v;
+                la(i, j, k, n) += 3.0 * kFourth * v;
+            } else {
+                if (cid >= 0) lcut[(int64_t)n * ncut + cid] = v;
+                else la(i, j, k, n) += v;
+            }
+        }
+    }
+}
+
+void adj_free(AdjState *A) {
+    if (!A) return;
+    if (A->D) hipFree(A->D);
+    for (int q = 0; q < 5; ++q) if (A->lam[q]) hipFree(A->lam[q]);
+    for (int q = 0; q < 3; ++q) {
+        if (A->cutv[q]) hipFree(A->cutv[q]);
+        if (A->lcut[q]) hipFree(A->lcut[q]);
+    }
+    if (A->grad) hipFree(A->grad);
+    if (A->J) hipFree(A->J);
+    for (double *b : A->bufs) hipFree(b);
+    delete A;
+}
+
+int adj_alloc(Level *L, int ncheck) {
+    Ctx *c = L->c;
+    AdjState *A = L->adj;
+    if (A && A->ncheck == ncheck && A->ncut == L->ncut) return MF_OK;
+    if (A) { adj_free(A); L->adj = nullptr; }
+    A = new (std::nothrow) AdjState();
+    if (!A) return MF_ERR_BAD_ARG;
+    L->adj = A;
+    A->ncheck = ncheck;
+    A->ncut = L->ncut;
+    const size_t pb = L->padded * sizeof(double);
+    const size_t cb = (size_t)std::max(L->ncut, 1) * c->nv * sizeof(double);
+    HIPCHK(hipMalloc(&A->D, pb));
+    for (int q = 0; q < 5; ++q) {
+        HIPCHK(hipMalloc(&A->lam[q], pb));
+        HIPCHK(hipMemset(A->lam[q], 0, pb));
+    }
+    for (int q = 0; q < 3; ++q) {
+        HIPCHK(hipMalloc(&A->cutv[q], cb));
+        HIPCHK(hipMalloc(&A->lcut[q], cb));
+    }
+    HIPCHK(hipMalloc(&A->grad, (MF_MAX_REACTIONS + 1) * sizeof(double)));
+    HIPCHK(hipMalloc(&A->J, sizeof(double)));
+    for (int q = 0; q < ncheck; ++q) {
+        double *b = nullptr;
+        HIPCHK(hipMalloc(&b, pb));
+        A->bufs.push_back(b);
+    }
+    return MF_OK;
+}
+
+// binomial split of the steps [a, b) with s free snapshots: the first
+// checkpoint after a (Griewank's beta(s, t) = C(s + t, s) rule)
+int adj_split(int a, int b, int s) {
+    const int l = b - a;
+    auto beta = [](int s_, int t_) -> double {
+        double v = 1.0;
+        for (int q = 1; q <= s_; ++q) v = v * (double)(t_ + q) / (double)q;
+        return v;
+    };
+    int t = 1;
+    while (beta(s, t) < (double)l) ++t;
+    const double right = beta(s - 1, t);
+    int left = (double)l > right ? (int)((double)l - right) : 1;
+    left = std::max(1, std::min(left, l - 1));
+    return a + left;
+}
+
+void adj_objective(Level *L) {
+    Ctx *c = L->c;
+    ObjArgs o;
+    o.tab = c->d_tab;
+    o.s = L->view[0];
+    o.ts = L->ts;
+    o.spec = L->adj->spec;
+    o.outlet_i = L->n[0] - 1;
+    o.ny = L->n[1];
+    o.nz = L->n[2];
+    o.dA = c->dx[1] * c->dx[2];
+    o.J = L->adj->J;
+    c->ks->adj->objective(o, c->stream);
+}
+
+void adj_copy(Level *L, double *dst, const double *src) {
+    const int64_t n = (int64_t)L->padded;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 16384)), dim3(256), 0, L->c->stream, dst, src, n);
+}
+void adj_zero(Level *L, double *p, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 16384)), dim3(256), 0, L->c->stream, p, n);
+}
+// U^step from its snapshot into A, time t0 + step dt
+void adj_restore(Level *L, int step) {
+    AdjState *A = L->adj;
+    adj_copy(L, L->arr[0], A->snaps[step]);
+    hipLaunchKernelGGL(k_set_ts, dim3(1), dim3(1), 0, L->c->stream, L->ts, A->t0 + step * A->dt, A->dt);
+}
+void adj_store(Level *L, int step) {
+    AdjState *A = L->adj;
+    double *b = A->pool.back();
+    A->pool.pop_back();
+    adj_copy(L, b, L->arr[0]);
+    A->snaps[step] = b;
+}
+void adj_release(Level *L, int step) {
+    AdjState *A = L->adj;
+    auto it = A->snaps.find(step);
+    if (it == A->snaps.end()) return;
+    A->pool.push_back(it->second);
+    A->snaps.erase(it);
+}
+
+// forward of one step keeping its stage arrays: A = U^n -> B = S1, C = S2,
+// D = S3 (post-flow, before the post-step pass), stage cut values in cutv[]
+void adj_forward_keep(Level *L) {
+    Ctx *c = L->c;
+    AdjState *A = L->adj;
+    for (int stage = 1; stage <= 3; ++stage) {
+        double *src = L->arr[stage - 1];
+        FillArgs f = fill_args(L, src, stage);
+        for (int ph = 0; ph <= 3; ++ph) c->ks->fill(f, ph, c->stream);
+        StageArgs a = stage_args(L, stage);
+        a.cutbuf = A->cutv[stage - 1];
+        if (L->ncut) {
+            CutArgs ca = cut_args(L, src);
+            ca.cutbuf = A->cutv[stage - 1];
+            c->ks->cut(ca, c->stream);
+        }
+        for (int d = 0; d < 3; ++d) launch_faces(L, a, d, MF_MODE_BATCHED);
+        ResArgs r;
+        std::memset(&r, 0, sizeof(r));
+        r.st = a;
+        r.combine = 1;
+        r.old = L->view[0];
+        r.dst = stage == 1 ? L->view[1] : (stage == 2 ? L->view[2] : padded_view(A->D, L->n, c->nv));
+        r.do_chem = c->prm.do_chemistry;
+        r.do_ns = c->prm.do_ns;
+        r.partials = L->partials;
+        launch_residual(L, r, MF_MODE_BATCHED);
+    }
+}
+
+// reverse of one stage: faces (3 sweeps), cut cells, ghost fill
+void adj_stage_reverse(Level *L, int stage, View lam_out, double coef, double *lam_in, bool zero_lcut) {
+    Ctx *c = L->c;
+    AdjState *A = L->adj;
+    const AdjSet *S = c->ks->adj;
+    double *src = L->arr[stage - 1];
+    View lin = padded_view(lam_in, L->n, c->nv);
+    if (zero_lcut && L->ncut) adj_zero(L, A->lcut[stage - 1], (int64_t)L->ncut * c->nv);
+    AdjFaceArgs fa;
+    fa.st = stage_args(L, stage);
+    fa.st.cutbuf = A->cutv[stage - 1];
+    fa.lam_out = lam_out;
+    fa.coef = coef;
+    fa.lam = lin;
+    fa.lcut = A->lcut[stage - 1];
+    for (int d = 0; d < 3; ++d) {
+        Work w = level_work(L, kAdjFaceClass + d);
+        const size_t shm = S->lds_bytes(L->tsh[kAdjFaceClass + d], d);
+        S->faces(fa, w, d, grid_for(L->ntiles[kAdjFaceClass + d], L->max_grid), 128, shm, c->stream);
+    }
+    if (L->ncut) {
+        AdjCutArgs ca;
+        ca.c = cut_args(L, src);
+        ca.c.cutbuf = A->cutv[stage - 1];
+        ca.lam = lin;
+        ca.lcut = A->lcut[stage - 1];
+        S->cut(ca, c->stream);
+    }
+    AdjFillArgs fl;
+    fl.f = fill_args(L, src, stage);
+    fl.lam = lin;
+    fl.grad = A->grad;
+    for (int ph = 3; ph >= 0; --ph) S->fill(fl, ph, c->stream);
+}
+
+// adjoint of step n: A holds U^n; lam[0] holds the adjoint of U^{n+1} on entry
+// and of U^n on exit
+void adj_step(Level *L, int n) {
+    Ctx *c = L->c;
+    AdjState *A = L->adj;
+    const AdjSet *S = c->ks->adj;
+    const double dt = A->dt;
+    hipLaunchKernelGGL(k_set_ts, dim3(1), dim3(1), 0, c->stream, L->ts, A->t0 + n * dt, dt);
+    adj_forward_keep(L);
+    View lu = padded_view(A->lam[0], L->n, c->nv), la = padded_view(A->lam[1], L->n, c->nv);
+    View lb = padded_view(A->lam[2], L->n, c->nv), lc = padded_view(A->lam[3], L->n, c->nv);
+    View ld = padded_view(A->lam[4], L->n, c->nv);
+    {
+        AdjPostArgs pa;
+        pa.st = stage_args(L, 3);
+        pa.st.src = padded_view(A->D, L->n, c->nv);
+        pa.lu = lu;
+        pa.ld = ld;
+        pa.clip = c->prm.clip_species;
+        pa.spec = A->spec;
+        pa.dA = c->dx[1] * c->dx[2];
+        pa.outlet_i = L->n[0] - 1;
+        pa.grad = A->grad;
+        Work w = level_work(L, kCellClass);
+        S->post(pa, w, grid_for(L->ntiles[kCellClass], L->max_grid), 256, c->stream);
+    }
+    const unsigned cg = (unsigned)std::min<int64_t>(((int64_t)L->n[0] * L->n[1] * L->n[2] + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_adj_comb, dim3(cg), dim3(256), 0, c->stream, 3, ld, la, lc, L->cutid, A->lcut[0], L->ncut, L->n[0],
+                       L->n[1], L->n[2], c->nv);
+    adj_stage_reverse(L, 3, ld, kTwo * dt * kThird, A->lam[3], true);
+    hipLaunchKernelGGL(k_adj_comb, dim3(cg), dim3(256), 0, c->stream, 2, lc, la, lb, L->cutid, A->lcut[0], L->ncut, L->n[0],
+                       L->n[1], L->n[2], c->nv);
+    adj_stage_reverse(L, 2, lc, dt * kFourth, A->lam[2], true);
+    hipLaunchKernelGGL(k_adj_comb, dim3(cg), dim3(256), 0, c->stream, 1, lb, la, lb, L->cutid, A->lcut[0], L->ncut, L->n[0],
+                       L->n[1], L->n[2], c->nv);
+    adj_stage_reverse(L, 1, lb, dt, A->lam[1], false);
+    std::swap(A->lam[0], A->lam[1]);
+}
+
+void adj_advance(Level *L, int from, int count, bool objective) {
+    for (int q = 0; q < count; ++q) {
+        enqueue_step(L, MF_MODE_BATCHED);
+        if (objective) adj_objective(L);
+    }
+    (void)from;
+}
+
+// reversal of the steps [a, b) with s free snapshot buffers; U^a is stored
+void adj_reverse(Level *L, int a, int b, int s) {
+    AdjState *A = L->adj;
+    if (b - a == 1) {
+        adj_restore(L, a);
+        adj_step(L, a);
+        return;
+    }
+    if (s <= 0) {
+        for (int st = b - 1; st >= a; --st) {
+            adj_restore(L, a);
+            adj_advance(L, a, st - a, false);
+            adj_step(L, st);
+        }
+        return;
+    }
+    const int m = adj_split(a, b, s);
+    if (!A->snaps.count(m)) {
+        adj_restore(L, a);
+        adj_advance(L, a, m - a, false);
+        adj_store(L, m);
+    }
+    adj_reverse(L, m, b, s - 1);
+    adj_release(L, m);
+    adj_reverse(L, a, m, s);
+}
+
 int read_ts(Level *L, TimeState *h) {
     HIPCHK(hipStreamSynchronize(L->c->stream));
     HIPCHK(hipMemcpy(h, L->ts, sizeof(TimeState), hipMemcpyDeviceToHost));
@@ -569,6 +877,21 @@ int ensure_scratch(Ctx *c, size_t bytes) {
     if (c->scratch) { HIPCHK(hipStreamSynchronize(c->stream)); hipFree(c->scratch); c->scratch = nullptr; }
     HIPCHK(hipMalloc(&c->scratch, bytes));
     c->scratch_bytes = bytes;
+    return MF_OK;
+}
+
+int upload_tab(Ctx *c, const mf_plugin_params *plugin) {
+    Tab ht;
+    std::memset(&ht, 0, sizeof(ht));
+    std::memcpy(static_cast<mf_plugin_params *>(&ht), plugin, sizeof(mf_plugin_params));
+    for (int q = 0; q < MF_MAX_SPECIES; ++q) {
+        ht.gm1[q] = plugin->gamma[q] - 1.0;
+        ht.cv[q] = plugin->R_univ / (plugin->gamma[q] - 1.0);
+        ht.cvm[q] = plugin->R_univ / (plugin->gamma[q] - 1.0) / plugin->molwt[q];
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(c->d_tab, &ht, sizeof(Tab), hipMemcpyHostToDevice));
+    c->tab = *plugin;
     return MF_OK;
 }
 
@@ -656,6 +979,14 @@ int mf_ctx_destroy(void *ctx) {
     if (c->scratch) hipFree(c->scratch);
     delete c;
     return MF_OK;
+}
+
+int mf_ctx_set_plugin_params(void *ctx, const mf_plugin_params *plugin) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c || !plugin) return MF_ERR_BAD_ARG;
+    if (plugin->kind != c->tab.kind || plugin->nspec != c->tab.nspec || plugin->ngas != c->tab.ngas) return MF_ERR_BAD_ARG;
+    if (plugin->nreact < 0 || plugin->nreact > MF_MAX_REACTIONS) return MF_ERR_BAD_ARG;
+    return upload_tab(c, plugin);
 }
 
 int mf_ctx_set_stream(void *ctx, void *stream) {
@@ -878,6 +1209,7 @@ int mf_level_destroy(void *level) {
     if (L->partials) hipFree(L->partials);
     if (L->d_box) hipFree(L->d_box);
     if (L->ts) hipFree(L->ts);
+    adj_free(L->adj);
     delete L;
     return MF_OK;
 }
@@ -1078,10 +1410,10 @@ int mf_level_set_tile(void *level, int cls, int tx, int ty, int tz) {
     Level *L = (Level *)level;
     if (!L || cls < 0 || cls >= kTileClasses) return MF_ERR_BAD_ARG;
     if (tx < 1 || ty < 1 || tz < 1 || (int64_t)tx * ty * tz > 4096) return MF_ERR_BAD_ARG;
-    if (cls < 3) {
-        int t[3] = {tx, ty, tz};
-        if (L->c->ks->lds_bytes(t, cls) > 160 * 1024) return MF_ERR_TOO_LARGE;
-    }
+    int t[3] = {tx, ty, tz};
+    if (cls < 3 && L->c->ks->lds_bytes(t, cls) > 160 * 1024) return MF_ERR_TOO_LARGE;
+    if (cls >= kAdjFaceClass && L->c->ks->adj && L->c->ks->adj->lds_bytes(t, cls - kAdjFaceClass) > 160 * 1024)
+        return MF_ERR_TOO_LARGE;
     HIPCHK(hipStreamSynchronize(L->c->stream));
     L->tsh[cls][0] = tx;
     L->tsh[cls][1] = ty;
@@ -1149,7 +1481,7 @@ int mf_level_autotune(void *level) {
         return ms;
     };
     const bool lds = use_lds(L);
-    for (int cls = 0; cls < kTileClasses && st == MF_OK; ++cls) {
+    for (int cls = 0; cls <= kCellClass && st == MF_OK; ++cls) {
         if (cls < 3 && !lds) continue;
         const int ncand = 8;
         double best = 1e300;
@@ -1196,12 +1528,92 @@ int mf_util_copy(void *ctx, double *dst, const double *src, int64_t n) {
 }
 
 int mf_adjoint_forward(void *level, int nsteps, int ncheck, int spec, double *J_out) {
-    (void)level; (void)nsteps; (void)ncheck; (void)spec; (void)J_out;
-    return MF_ERR_UNSUPPORTED;
+    Level *L = (Level *)level;
+    if (!L || nsteps < 1 || ncheck < 1) return MF_ERR_BAD_ARG;
+    Ctx *c = L->c;
+    if (spec < 0 || spec >= c->ns) return MF_ERR_BAD_ARG;
+    if (!c->ks->adj || !c->prm.fixed_dt || !(c->prm.dt_fixed > 0) || c->prm.do_ns != 1 || c->prm.order_hyp != 5 ||
+        c->prm.conj_ht != 0 || c->prm.do_chemistry != 1)
+        return MF_ERR_UNSUPPORTED;
+    ncheck = std::min(ncheck, nsteps);
+    int st = adj_alloc(L, ncheck);
+    if (st) return st;
+    AdjState *A = L->adj;
+    A->nsteps = nsteps;
+    A->spec = spec;
+    A->ready = false;
+    A->snaps.clear();
+    A->pool = A->bufs;
+    TimeState h;
+    st = read_ts(L, &h);
+    if (st) return st;
+    A->t0 = h.t;
+    A->dt = c->prm.dt_fixed;
+    HIPCHK(hipMemset(A->J, 0, sizeof(double)));
+    hipLaunchKernelGGL(k_set_ts, dim3(1), dim3(1), 0, c->stream, L->ts, A->t0, A->dt);
+    adj_store(L, 0);
+    // the right spine of the binomial reversal is stored on the way
+    int pos = 0, s = ncheck - 1;
+    while (nsteps - pos > 1 && s > 0) {
+        const int m = adj_split(pos, nsteps, s);
+        adj_advance(L, pos, m - pos, true);
+        adj_store(L, m);
+        pos = m;
+        --s;
+    }
+    adj_advance(L, pos, nsteps - pos, true);
+    HIPCHK(hipGetLastError());
+    st = read_ts(L, &h);
+    if (st) return st;
+    if (h.err) return h.err;
+    double J = 0.0;
+    HIPCHK(hipMemcpy(&J, A->J, sizeof(double), hipMemcpyDeviceToHost));
+    if (J_out) *J_out = J;
+    A->ready = true;
+    return MF_OK;
 }
+
 int mf_adjoint_backward(void *level, double *grad_logA, double *grad_invel) {
-    (void)level; (void)grad_logA; (void)grad_invel;
-    return MF_ERR_UNSUPPORTED;
+    Level *L = (Level *)level;
+    if (!L || !L->adj || !L->adj->ready) return MF_ERR_BAD_ARG;
+    Ctx *c = L->c;
+    AdjState *A = L->adj;
+    HIPCHK(hipMemset(A->grad, 0, (MF_MAX_REACTIONS + 1) * sizeof(double)));
+    for (int q = 0; q < 5; ++q) HIPCHK(hipMemset(A->lam[q], 0, L->padded * sizeof(double)));
+    adj_reverse(L, 0, A->nsteps, A->ncheck - 1);
+    hipLaunchKernelGGL(k_set_ts, dim3(1), dim3(1), 0, c->stream, L->ts, A->t0, A->dt);
+    HIPCHK(hipGetLastError());
+    TimeState h;
+    int st = read_ts(L, &h);
+    if (st) return st;
+    double g[MF_MAX_REACTIONS + 1];
+    HIPCHK(hipMemcpy(g, A->grad, sizeof(g), hipMemcpyDeviceToHost));
+    if (grad_logA) for (int r = 0; r < c->tab.nreact; ++r) grad_logA[r] = g[r];
+    if (grad_invel) *grad_invel = g[c->tab.nreact];
+    A->ready = false;
+    if (h.err) return h.err;
+    return MF_OK;
+}
+
+int mf_adjoint_state_grad(void *level, double *out, int is_device) {
+    Level *L = (Level *)level;
+    if (!L || !out || !L->adj) return MF_ERR_BAD_ARG;
+    Ctx *c = L->c;
+    const int64_t ncell = (int64_t)L->n[0] * L->n[1] * L->n[2];
+    const size_t bytes = (size_t)ncell * c->nv * sizeof(double);
+    double *dst = out, *tmp = nullptr;
+    if (!is_device) {
+        HIPCHK(hipMalloc(&tmp, bytes));
+        dst = tmp;
+    }
+    hipLaunchKernelGGL(k_pack, dim3(4096), dim3(256), 0, c->stream, dst, padded_view(L->adj->lam[0], L->n, c->nv), L->n[0], L->n[1],
+                       L->n[2], c->nv, 0);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (tmp) {
+        HIPCHK(hipMemcpy(out, tmp, bytes, hipMemcpyDeviceToHost));
+        hipFree(tmp);
+    }
+    return MF_OK;
 }
 
 }  // extern "C"

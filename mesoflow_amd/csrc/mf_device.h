@@ -9,7 +9,7 @@
 #include <stdint.h>
 #include "mesoflow_amd.h"
 
-#define MFD __device__ __forceinline__
+#define MFD __host__ __device__ __forceinline__
 
 namespace mfd {
 

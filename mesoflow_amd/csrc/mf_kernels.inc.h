@@ -1194,9 +1194,10 @@ struct Launch {
     static void est(const ResArgs &r, const Work &w, int grid, int block, hipStream_t st) {
         hipLaunchKernelGGL(k_est<P>, dim3(grid), dim3(block), 0, st, r, w);
     }
-    static KernelSet set() {
+    static KernelSet set(const AdjSet *adj = nullptr) {
         KernelSet k;
         k.ns = P::NS;
+        k.adj = adj;
         k.fill = &fill;
         k.cut = &cut;
         k.cut_box = &cut_box;

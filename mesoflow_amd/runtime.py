@@ -23,6 +23,7 @@ SIGNATURES = {
     "mf_device_count": ([_P(C.c_int)], C.c_int),
     "mf_ctx_create": ([_P(abi.PluginParams), _P(abi.Params), _P(abi.Geom), _P(C.c_void_p)], C.c_int),
     "mf_ctx_destroy": ([C.c_void_p], C.c_int),
+    "mf_ctx_set_plugin_params": ([C.c_void_p, C.c_void_p], C.c_int),
     "mf_ctx_set_stream": ([C.c_void_p, C.c_void_p], C.c_int),
     "mf_ctx_nvars": ([C.c_void_p, _P(C.c_int)], C.c_int),
     "mf_ctx_sync": ([C.c_void_p], C.c_int),
@@ -45,6 +46,7 @@ SIGNATURES = {
     "mf_level_autotune": ([C.c_void_p], C.c_int),
     "mf_adjoint_forward": ([C.c_void_p, C.c_int, C.c_int, C.c_int, _P(C.c_double)], C.c_int),
     "mf_adjoint_backward": ([C.c_void_p, C.c_void_p, _P(C.c_double)], C.c_int),
+    "mf_adjoint_state_grad": ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),
     "mf_util_copy": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64], C.c_int),
 }
 
@@ -105,6 +107,10 @@ class Context:
 
     def set_stream(self, stream_ptr):
         check(lib().mf_ctx_set_stream(self.h, C.c_void_p(stream_ptr)), "mf_ctx_set_stream")
+
+    def set_plugin_params(self, plugin):
+        """Replace the plugin table (same family and species counts)."""
+        check(lib().mf_ctx_set_plugin_params(self.h, C.byref(plugin)), "mf_ctx_set_plugin_params")
 
 
 class Level:
@@ -219,6 +225,30 @@ class Level:
     def autotune(self):
         check(lib().mf_level_autotune(self.h), "mf_level_autotune")
         return self.tiles()
+
+
+    # ----- discrete adjoint ----------------------------------------------------
+    def adjoint_forward(self, nsteps, ncheck, spec):
+        """Run nsteps fixed-dt steps keeping binomial checkpoints; returns the
+        objective J (time-integrated outlet mass flux of species `spec`)."""
+        J = C.c_double()
+        check(lib().mf_adjoint_forward(self.h, int(nsteps), int(ncheck), int(spec), C.byref(J)), "mf_adjoint_forward")
+        return J.value
+
+    def adjoint_backward(self):
+        """Reverse sweep of the last adjoint_forward: (dJ/dlog_A[nreact], dJ/din_vel)."""
+        nr = int(self.ctx.case.plugin.nreact)
+        g = np.zeros(max(nr, 1), dtype=np.float64)
+        gv = C.c_double()
+        check(lib().mf_adjoint_backward(self.h, g.ctypes.data_as(C.c_void_p), C.byref(gv)), "mf_adjoint_backward")
+        return g[:nr].copy(), gv.value
+
+    def adjoint_state_grad(self):
+        """dJ/d(initial state) of the last backward sweep, (nv, nz, ny, nx)."""
+        nx, ny, nz = self.ctx.case.n_cell
+        out = np.empty((self.ctx.nv, nz, ny, nx), dtype=np.float64)
+        check(lib().mf_adjoint_state_grad(self.h, out.ctypes.data_as(C.c_void_p), 0), "mf_adjoint_state_grad")
+        return out
 
 
 _HIPMEMCPY = None
