@@ -93,6 +93,71 @@ def cpu_baseline(name, budget_s):
             "sample": "%s, %d RK3 steps after 1 warm-up step, %.1f s, OMP threads %d" % (sample, nmore, t3 - t2, threads)}
 
 
+def adjoint_bench(nsteps=200, ncheck_binomial=20):
+    """Discrete adjoint of the porous reactor at 128^3 (5 species, 3 reactions,
+    sphere pack): time of the plain forward per step, of mf_adjoint_forward and
+    of the reverse sweep (mf_adjoint_backward) per step, with every state kept
+    (the schedule degenerates to store-all when the snapshots fit in HBM) and
+    with ncheck_binomial binomial checkpoints (forward segments recomputed)."""
+    import torch
+    from mesoflow_amd import abi, cases, runtime
+    c = cases.porous_reactor(n_cell=(128, 128, 128), nspec=5, nreact=3, geometry="spheres", porosity=0.6, rmin=6.0,
+                             rmax=14.0, order_hyp=5, cfl=0.3, seed=20261205)
+    s0 = c.make_state()
+    ctx0 = runtime.Context(c)
+    L0 = runtime.Level(ctx0)
+    L0.set_state(s0)
+    dt = L0.est_dt()
+    L0.close()
+    ctx0.close()
+    c.params.fixed_dt = 1
+    c.params.dt_fixed = dt
+    ctx = runtime.Context(c)
+    L = runtime.Level(ctx)
+    L.set_state(s0)
+    L.autotune()
+
+    def fresh():
+        L.set_state(s0)
+        L.set_time(0.0, 1.0e100, 0)
+        torch.cuda.synchronize()
+
+    fresh()
+    L.step(2, abi.MODE_BATCHED)
+    fresh()
+    t0 = time.perf_counter()
+    L.step(nsteps, abi.MODE_BATCHED)
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) / nsteps
+    out = {"workload": "porous reactor 128^3 FP64, 5 species, 3 Arrhenius reactions, IB sphere pack r~U[6,14] porosity 0.6, "
+                       "WENO5, fixed dt, %d RK3 steps; J = time-integrated outlet mass flux of species 1; gradient "
+                       "w.r.t. the 3 log pre-exponentials and the inflow velocity" % nsteps,
+           "forward_ms_per_step": fwd * 1e3}
+    padded = (128 + 6) ** 3 * c.nv * 8
+    free, _ = torch.cuda.mem_get_info()
+    runs = []
+    if nsteps * padded < 0.6 * free:
+        runs.append(("store_all", nsteps))
+    runs.append(("binomial", ncheck_binomial))
+    for name, ncheck in runs:
+        fresh()
+        t0 = time.perf_counter()
+        J = L.adjoint_forward(nsteps, ncheck, 1)
+        t1 = time.perf_counter()
+        gA, gv = L.adjoint_backward()
+        t2 = time.perf_counter()
+        bwd = (t2 - t1) / nsteps
+        out[name] = {"ncheck": ncheck, "adjoint_forward_ms_per_step": (t1 - t0) / nsteps * 1e3,
+                     "backward_ms_per_step": bwd * 1e3, "backward_over_forward": bwd / fwd,
+                     "J": J, "dJ_dlogA": [float(x) for x in gA], "dJ_dinvel": gv}
+        log("adjoint %s: %s" % (name, json.dumps(out[name])))
+    best = min(out[n]["backward_over_forward"] for n, _ in runs)
+    out["adjoint_over_forward"] = best
+    L.close()
+    ctx.close()
+    return out
+
+
 def read_traffic(config):
     path = os.path.join(ROOT, "profiles", "traffic_%s.txt" % config)
     if not os.path.exists(path):
@@ -117,6 +182,8 @@ def main():
     ap.add_argument("--calibrate", action="store_true", help="plain copy kernel for PMC scale factors")
     ap.add_argument("--compare-steps", type=int, default=3,
                     help="steps timed in each launch mode after the main region (0 = skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="steps of the discrete-adjoint measurement at 128^3 (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -197,6 +264,7 @@ def main():
         extra["kernel_time_share"] = {k: round(v[0] / tot, 4) for k, v in classes.items()}
     extra["step_hbm_frac"] = (step_bytes * args.steps / secs / 1e9) / HBM_PEAK_GBS
     extra["tile"] = ";".join(",".join(str(v) for v in t) for t in lvl.tiles())
+    nboxes = lvl.nboxes
 
     if args.compare_steps > 0:
         # the same steps launched per box (the reference's shape), as persistent
@@ -214,6 +282,14 @@ def main():
         extra["batched_speedup_vs_per_box"] = modes["per_box"] / modes["batched"]
         extra["graph_speedup_vs_per_box"] = modes["per_box"] / modes["graph"]
 
+    if args.adjoint_steps > 0:
+        lvl.close()
+        ctx.close()
+        lvl = ctx = None
+        torch.cuda.empty_cache()
+        log("discrete adjoint at 128^3 ...")
+        extra["adjoint"] = adjoint_bench(args.adjoint_steps)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU oracle on a sample ...")
@@ -226,12 +302,13 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: seeded initial state of the named workload (no datasets)",
         "config": {"workload": desc, "cells": cells, "nspec": case.nspec, "nv": nv, "order_hyp": int(case.params.order_hyp),
-                   "mode": args.mode, "boxes": lvl.nboxes},
+                   "mode": args.mode, "boxes": nboxes},
         "roofline": roof, "cpu_baseline": cpu, "extra": extra,
     }
     print(json.dumps(out), flush=True)
-    lvl.close()
-    ctx.close()
+    if lvl is not None:
+        lvl.close()
+        ctx.close()
 
 
 if __name__ == "__main__":
