@@ -239,7 +239,7 @@ __global__ __launch_bounds__(128) void k_cut_adj(AdjCutArgs aa) {
     constexpr int NV = Num<P>::NV;
     const CutArgs &a = aa.c;
     if (step_skipped(a.ts)) return;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (q >= a.ncut) return;
     double ob[NV];
 #pragma unroll
@@ -445,7 +445,9 @@ struct AdjLaunch {
     }
     static void cut(const AdjCutArgs &a, hipStream_t st) {
         if (a.c.ncut <= 0) return;
-        hipLaunchKernelGGL(k_cut_adj<P>, dim3((a.c.ncut + 127) / 128), dim3(128), 0, st, a);
+        int g = (a.c.ncut + 127) / 128;
+        if (g >= 8) g = (g + 7) & ~7;
+        hipLaunchKernelGGL(k_cut_adj<P>, dim3(g), dim3(128), 0, st, a);
     }
     static void fill(const AdjFillArgs &a, int phase, hipStream_t st) {
         const int G = MF_NGROW;

@@ -71,7 +71,6 @@ struct ResArgs {
     View dsdt;                 // combine == 0 (single box)
     View old;                  // U^n (S_old)
     View dst;                  // output state of the stage
-    int epilogue;              // stage 3: chemistry, primitives, clip, residual, dt partials
     int do_chem, clip, track_res, do_ns;
     double *partials;          // [tile][nv + 2]
 };

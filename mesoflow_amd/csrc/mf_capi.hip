@@ -94,6 +94,7 @@ struct Level {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     struct AdjState *adj = nullptr;
+    int post_slots = 1;  // partial-sum slots written by the post-step pass (one per block)
 };
 
 // Discrete adjoint of a fixed-dt run of nsteps steps (mf_adjoint_forward /
@@ -421,16 +422,28 @@ int rebuild_tiles(Level *L) {
 int build_cutlist(Level *L, const double *vfrac_host) {
     const int nx = L->n[0], ny = L->n[1], nz = L->n[2];
     const int64_t ncell = (int64_t)nx * ny * nz;
+    // cut cells ordered by 8^3 bricks (raster order inside a brick), so that a
+    // run of the list covers a compact region and its IDW neighbourhoods overlap
     std::vector<int> id(ncell, -1), li, lj, lk;
-    int nc = 0;
+    std::vector<std::pair<int64_t, int64_t>> order;
+    const int B = 8;
+    const int64_t nbx = (nx + B - 1) / B, nby = (ny + B - 1) / B;
     for (int64_t q = 0; q < ncell; ++q) {
         double v = vfrac_host[q];
         if (v > 0.0 && v < 1.0) {
-            id[q] = nc++;
-            li.push_back((int)(q % nx));
-            lj.push_back((int)((q / nx) % ny));
-            lk.push_back((int)(q / ((int64_t)nx * ny)));
+            const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((int64_t)nx * ny));
+            const int64_t brick = ((int64_t)(k / B) * nby + j / B) * nbx + i / B;
+            order.push_back({brick, q});
         }
+    }
+    std::sort(order.begin(), order.end());
+    int nc = 0;
+    for (const auto &e : order) {
+        const int64_t q = e.second;
+        id[q] = nc++;
+        li.push_back((int)(q % nx));
+        lj.push_back((int)((q / nx) % ny));
+        lk.push_back((int)(q / ((int64_t)nx * ny)));
     }
     if (L->cutlist) { hipFree(L->cutlist); L->cutlist = nullptr; }
     if (L->cutbuf) { hipFree(L->cutbuf); L->cutbuf = nullptr; }
@@ -524,7 +537,6 @@ void enqueue_stage(Level *L, int stage, int mode) {
     // stage 3 writes A in place with chemistry; without, B keeps S3 so that the
     // post pass still sees U^n in A for the residual
     r.dst = L->view[stage == 1 ? 1 : (stage == 2 ? 2 : (c->prm.do_chemistry ? 0 : 1))];
-    r.epilogue = 0;
     r.do_chem = c->prm.do_chemistry;
     r.clip = c->prm.clip_species;
     r.track_res = c->prm.track_residual_norms;
@@ -545,8 +557,10 @@ void enqueue_stage(Level *L, int stage, int mode) {
                 w.t1 = w.t0 + box_tiles(bd, kCellClass);
                 c->ks->post(r, w, w.t1 - w.t0, 256, c->stream);
             }
+            L->post_slots = L->ntiles[kCellClass];
         } else {
-            c->ks->post(r, w, grid_for(L->ntiles[kCellClass], L->max_grid), 256, c->stream);
+            L->post_slots = grid_for(L->ntiles[kCellClass], L->max_grid);
+            c->ks->post(r, w, L->post_slots, 256, c->stream);
         }
         L->timer.end(c->stream);
     }
@@ -560,7 +574,7 @@ void enqueue_step(Level *L, int mode) {
     L->timer.end(c->stream);
     for (int s = 1; s <= 3; ++s) enqueue_stage(L, s, mode);
     L->timer.begin(KC_FINAL, c->stream);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, L->partials, L->ntiles[kCellClass], c->nv, L->ts, c->d_err, 0,
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, L->partials, L->post_slots, c->nv, L->ts, c->d_err, 0,
                        c->prm.track_residual_norms);
     L->timer.end(c->stream);
 }
@@ -574,8 +588,9 @@ void enqueue_est(Level *L) {
     r.do_ns = c->prm.do_ns;
     r.partials = L->partials;
     Work w = level_work(L, kCellClass);
-    c->ks->est(r, w, std::min(L->ntiles[kCellClass], L->max_grid), 256, c->stream);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, L->partials, L->ntiles[kCellClass], c->nv, L->ts, c->d_err, 1, 0);
+    const int grid = std::min(L->ntiles[kCellClass], L->max_grid);
+    c->ks->est(r, w, grid, 256, c->stream);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, L->partials, grid, c->nv, L->ts, c->d_err, 1, 0);
 }
 
 // ---------------------------------------------------------------------------

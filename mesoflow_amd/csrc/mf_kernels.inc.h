@@ -370,23 +370,73 @@ __global__ __launch_bounds__(256) void k_faces(StageArgs a, Work w) {
 // ---------------------------------------------------------------------------
 // WENO5 face sweep of direction D with the stencil staged in LDS. A tile of
 // faces first stages every stencil cell once (conserved state, the velocities
-// and pressure of geteulerflux, gamma and vfrac); each face then reads its six
-// cells from LDS. Same arithmetic as face_flux<P,5,D>; the plugin's gamma must
-// depend on the species only (then gamma(rho_R, presR, spec_R) of the
-// eigenvalue equals the staged gamma of the right cell).
+// and pressure of geteulerflux, gamma, vfrac and the cell's cut-cell index);
+// each face then reads its six cells from LDS. Per face the viscous stress and
+// work, conduction and species diffusion terms are formed first and kept in
+// registers, then each flux component is reconstructed and written once with
+// those terms subtracted in the reference's order. Same arithmetic as
+// face_flux<P,5,D>; the plugin's gamma must depend on the species only (then
+// gamma(rho_R, presR, spec_R) of the eigenvalue equals the staged gamma).
 // ---------------------------------------------------------------------------
+// component n of cell (i,j,k) as a box sees it: the array value, loaded
+// independently of the cut-index load, replaced for an own-box cut cell
+MFD double sel_get(const Acc &acc, int i, int j, int k, int n, int cid) {
+    double v = acc.s(i, j, k, n);
+    if (cid >= 0) v = acc.cutbuf[(int64_t)n * acc.ncut + cid];
+    return v;
+}
+// convective flux h of component c with the viscous, conduction and species
+// diffusion terms of face_pass2 subtracted in the same order
+MFD double flux_with_terms(int c, double h, bool hyp, double dm0, double dm1, double dm2, double de0, double de1, double de2,
+                           double de3, double dsp) {
+    double v = h;
+    if (c >= RHOU && c <= RHOW) {
+        if (hyp) v = v - (c == RHOU ? dm0 : (c == RHOV ? dm1 : dm2));
+    } else if (c == RHOE) {
+        if (hyp) {
+            v -= de0;
+            v -= de1;
+            v -= de2;
+        }
+        v = v - de3;
+    } else if (c >= NCV) {
+        v = v - dsp;
+    }
+    return v;
+}
+// a staged cell seen by a pointwise plugin: conserved components, species and
+// vfrac from the LDS tile, the primitives from memory
+template <int NC>
+struct LdsCell {
+    const double *lds;
+    int ncs, q;
+    const Acc *acc;
+    int i, j, k, cid;
+    MFD double operator()(int n) const {
+        if (n < NCV) return lds[n * ncs + q];
+        if (n >= FLO) return lds[(NCV + n - FLO) * ncs + q];
+        if (n == VFRAC) return lds[(NC + 5) * ncs + q];
+        return sel_get(*acc, i, j, k, n, cid);
+    }
+};
+
+// minimum waves per SIMD the compiler must keep room for (register budget)
+#ifndef MF_FACE_MIN_WAVES
+#define MF_FACE_MIN_WAVES 3
+#endif
 template <class P, int D>
-__global__ __launch_bounds__(256) void k_faces_lds(StageArgs a, Work w) {
+__global__ __launch_bounds__(256, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs a, Work w) {
     using N = Num<P>;
     constexpr int NS = P::NS, NC = N::NC;
     constexpr int e0 = D == 0, e1 = D == 1, e2 = D == 2;
-    constexpr int FVX = NC, FP = NC + 3, FG = NC + 4, FVF = NC + 5;
+    constexpr int T1 = (D + 1) % 3, T2 = (D + 2) % 3;
+    constexpr int FVX = NC, FP = NC + 3, FG = NC + 4, FVF = NC + 5, NF = NC + 6;
     static_assert(P::kGammaIgnoresRhoP, "staged gamma needs a species-only gamma");
     extern __shared__ double lds[];
     if (step_skipped(a.ts)) return;
     const Tab &T = *a.tab;
-    const bool ns = a.nsflag != 0;
     const double dxs = a.dx[D];
+    const int full = a.flux_full;
     for (TileIter it(w); it.ok(); it.next()) {
         const int t = it.t;
         int lt;
@@ -398,28 +448,40 @@ __global__ __launch_bounds__(256) void k_faces_lds(StageArgs a, Work w) {
         const int sx = fx + 5 * e0, sy = fy + 5 * e1, sz = fz + 5 * e2;
         const int ncs = sx * sy * sz;
         const int ox = c0[0] - 3 * e0, oy = c0[1] - 3 * e1, oz = c0[2] - 3 * e2;
+        int *lcid = reinterpret_cast<int *>(lds + NF * ncs);
         Acc acc = make_acc(a, b);
-        if (ns) {
-            for (int q = threadIdx.x; q < ncs; q += blockDim.x) {
-                const int ix = q % sx, r = q / sx, iy = r % sy, iz = r / sy;
-                const int ci = ox + ix, cj = oy + iy, ck = oz + iz;
-                const CellPtr cp = cell_ptr(acc, ci, cj, ck, acc.cid(ci, cj, ck));
-                double u[NC];
+        for (int q = threadIdx.x; q < ncs; q += blockDim.x) {
+            const int ix = q % sx, r = q / sx, iy = r % sy, iz = r / sy;
+            const int ci = ox + ix, cj = oy + iy, ck = oz + iz;
+            const int cid = acc.cid(ci, cj, ck);
+            const double *sp = acc.s.p + acc.s.off(ci, cj, ck);
+            const int64_t ns = acc.s.ns;
+            double u[NC];
 #pragma unroll
-                for (int n = 0; n < NCV; ++n) u[n] = cp(n);
+            for (int n = 0; n < NCV; ++n) u[n] = sp[(int64_t)n * ns];
 #pragma unroll
-                for (int n = 0; n < NS; ++n) u[NCV + n] = cp(FLO + n);
+            for (int n = 0; n < NS; ++n) u[NCV + n] = sp[(int64_t)(FLO + n) * ns];
+            double vf = sp[(int64_t)VFRAC * ns];
+            if (cid >= 0) {
+                const double *cp = acc.cutbuf + cid;
+                const int64_t cs = acc.ncut;
 #pragma unroll
-                for (int c = 0; c < NC; ++c) lds[c * ncs + q] = u[c];
-                double vx, vy, vz, p;
-                N::euler_prims(T, u, vx, vy, vz, p);
-                lds[(FVX + 0) * ncs + q] = vx;
-                lds[(FVX + 1) * ncs + q] = vy;
-                lds[(FVX + 2) * ncs + q] = vz;
-                lds[FP * ncs + q] = p;
-                lds[FG * ncs + q] = P::get_gama_from_rpc(T, u[RHO], p, u + NCV);
-                lds[FVF * ncs + q] = cp(VFRAC);
+                for (int n = 0; n < NCV; ++n) u[n] = cp[(int64_t)n * cs];
+#pragma unroll
+                for (int n = 0; n < NS; ++n) u[NCV + n] = cp[(int64_t)(FLO + n) * cs];
+                vf = cp[(int64_t)VFRAC * cs];
             }
+#pragma unroll
+            for (int c = 0; c < NC; ++c) lds[c * ncs + q] = u[c];
+            double vx, vy, vz, p;
+            N::euler_prims(T, u, vx, vy, vz, p);
+            lds[(FVX + 0) * ncs + q] = vx;
+            lds[(FVX + 1) * ncs + q] = vy;
+            lds[(FVX + 2) * ncs + q] = vz;
+            lds[FP * ncs + q] = p;
+            lds[FG * ncs + q] = P::get_gama_from_rpc(T, u[RHO], p, u + NCV);
+            lds[FVF * ncs + q] = vf;
+            lcid[q] = cid;
         }
         __syncthreads();
         const int nf = fx * fy * fz;
@@ -430,31 +492,111 @@ __global__ __launch_bounds__(256) void k_faces_lds(StageArgs a, Work w) {
             const int q0 = ix + sx * (iy + sy * iz);
             auto Ld = [&](int field, int m) { return lds[field * ncs + q0 + m * sD]; };
             const int iL = i - e0, jL = j - e1, kL = k - e2;
-            const int cL = acc.cid(iL, jL, kL), cR = acc.cid(i, j, k);
-            const CellPtr pL = cell_ptr(acc, iL, jL, kL, cL), pR = cell_ptr(acc, i, j, k, cR);
+            const int qL = q0 + 2 * sD, qR = q0 + 3 * sD;
+            const int cL = lcid[qL], cR = lcid[qR];
+            const double vfL = Ld(FVF, 2), vfR = Ld(FVF, 3);
+            const double facevfrac = kHalf * (vfL + vfR);
+            const bool hyp = facevfrac >= 0.5;
+            // ---- viscous stress and work (:293-367), conduction (:373-374), species diffusion (:425-428)
+            double dm0 = 0.0, dm1 = 0.0, dm2 = 0.0, de0 = 0.0, de1 = 0.0, de2 = 0.0, de3, dsp[NS];
+            {
+                double visc, thcond;
+                {
+                    // harmonic mean of (i,j,k) and (i-1,j,k) for every sweep direction (:179-180)
+                    double v0, k0, v1, k1;
+                    LdsCell<NC> c0c{lds, ncs, qR, &acc, i, j, k, cR};
+                    P::fluid_transport(T, c0c, v0, k0);
+                    if (D == 0) {
+                        LdsCell<NC> c1c{lds, ncs, qL, &acc, iL, jL, kL, cL};
+                        P::fluid_transport(T, c1c, v1, k1);
+                    } else {
+                        CellRef c1c{&acc, i - 1, j, k, acc.cid(i - 1, j, k)};
+                        P::fluid_transport(T, c1c, v1, k1);
+                    }
+                    visc = N::harmonic_mean(v0, v1);
+                    thcond = N::harmonic_mean(k0, k1);
+                }
+                if (hyp) {
+                    constexpr int p10 = T1 == 0, p11 = T1 == 1, p12 = T1 == 2;
+                    constexpr int p20 = T2 == 0, p21 = T2 == 1, p22 = T2 == 2;
+                    const int tc0 = acc.cid(iL + p10, jL + p11, kL + p12), tc1 = acc.cid(i + p10, j + p11, k + p12);
+                    const int tc2 = acc.cid(iL - p10, jL - p11, kL - p12), tc3 = acc.cid(i - p10, j - p11, k - p12);
+                    const int tc4 = acc.cid(iL + p20, jL + p21, kL + p22), tc5 = acc.cid(i + p20, j + p21, k + p22);
+                    const int tc6 = acc.cid(iL - p20, jL - p21, kL - p22), tc7 = acc.cid(i - p20, j - p21, k - p22);
+                    double g[3][3];  // g[q][m]: d(vel q) along normal (m=0), t1 (m=1), t2 (m=2)
+                    double vL[3], vR[3];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const int V = VELX + q;
+                        vL[q] = sel_get(acc, iL, jL, kL, V, cL);
+                        vR[q] = sel_get(acc, i, j, k, V, cR);
+                        g[q][0] = (vR[q] - vL[q]) / dxs;
+                        double top = kHalf * (sel_get(acc, iL + p10, jL + p11, kL + p12, V, tc0) +
+                                              sel_get(acc, i + p10, j + p11, k + p12, V, tc1));
+                        double bot = kHalf * (sel_get(acc, iL - p10, jL - p11, kL - p12, V, tc2) +
+                                              sel_get(acc, i - p10, j - p11, k - p12, V, tc3));
+                        g[q][1] = (top - bot) / (kTwo * a.dx[T1]);
+                        double frt = kHalf * (sel_get(acc, iL + p20, jL + p21, kL + p22, V, tc4) +
+                                              sel_get(acc, i + p20, j + p21, k + p22, V, tc5));
+                        double bck = kHalf * (sel_get(acc, iL - p20, jL - p21, kL - p22, V, tc6) +
+                                              sel_get(acc, i - p20, j - p21, k - p22, V, tc7));
+                        g[q][2] = (frt - bck) / (kTwo * a.dx[T2]);
+                    }
+                    // axis -> gradient slot (GET_XDIR/GET_YDIR/GET_ZDIR, globalDefines.H:49-51)
+                    constexpr int sx_ = D == 0 ? 0 : (T1 == 0 ? 1 : 2);
+                    constexpr int sy_ = D == 1 ? 0 : (T1 == 1 ? 1 : 2);
+                    constexpr int sz_ = D == 2 ? 0 : (T1 == 2 ? 1 : 2);
+                    double dudx = g[0][sx_], dudy = g[0][sy_], dudz = g[0][sz_];
+                    double dvdx = g[1][sx_], dvdy = g[1][sy_], dvdz = g[1][sz_];
+                    double dwdx = g[2][sx_], dwdy = g[2][sy_], dwdz = g[2][sz_];
+                    double delu = dudx + dvdy + dwdz;
+                    double t0, t1, t2;
+                    if (D == 0) {
+                        t0 = visc * (kTwo * dudx - kTwo3rd * delu);
+                        t1 = visc * (dudy + dvdx);
+                        t2 = visc * (dudz + dwdx);
+                    } else if (D == 1) {
+                        t0 = visc * (dvdx + dudy);
+                        t1 = visc * (kTwo * dvdy - kTwo3rd * delu);
+                        t2 = visc * (dvdz + dwdy);
+                    } else {
+                        t0 = visc * (dwdx + dudz);
+                        t1 = visc * (dwdy + dvdz);
+                        t2 = visc * (kTwo * dwdz - kTwo3rd * delu);
+                    }
+                    dm0 = t0;
+                    dm1 = t1;
+                    dm2 = t2;
+                    de0 = t0 * (kHalf * (vL[0] + vR[0]));
+                    de1 = t1 * (kHalf * (vL[1] + vR[1]));
+                    de2 = t2 * (kHalf * (vL[2] + vR[2]));
+                }
+                const double dTdn = (sel_get(acc, i, j, k, TEMP, cR) - sel_get(acc, iL, jL, kL, TEMP, cL)) / dxs;
+                de3 = thcond * dTdn;
+                double dl[NS], dr[NS];
+                LdsCell<NC> lc{lds, ncs, qL, &acc, iL, jL, kL, cL};
+                LdsCell<NC> rc{lds, ncs, qR, &acc, i, j, k, cR};
+                P::spec_dcoeff(T, lc, dl);
+                P::spec_dcoeff(T, rc, dr);
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    double Ds = N::harmonic_mean(dl[s], dr[s]);
+                    dsp[s] = Ds * (lds[(NCV + s) * ncs + qR] - lds[(NCV + s) * ncs + qL]) / dxs;
+                }
+            }
             const View &F = b.flux[D];
-            const int full = a.flux_full;
             double *Fp = F.p + F.off(i, j, k);
             const int64_t Fs = F.ns;
-            auto fput = [&](int c, double v) { Fp[(int64_t)flux_comp(c, full) * Fs] = v; };
-            const double vfL = ns ? Ld(FVF, 2) : pL(VFRAC), vfR = ns ? Ld(FVF, 3) : pR(VFRAC);
-            const double facevfrac = kHalf * (vfL + vfR);
-            const bool hyp = ns && facevfrac >= 0.5;
-            if (!ns) {
-                double velface = kHalf * (pL(VELX + D) + pR(VELX + D));
-                if (a.spec_in_solid) {
-                    double wl = (vfL < kOne) ? kZero : kOne;
-                    velface = (wl * pL(VELX + D) + wl * pR(VELX + D)) / (wl + wl + kTiny);
+            // species diffusion terms parked in the flux slots; the component
+            // loop below subtracts them from the convective flux
+#pragma unroll
+            for (int s = 0; s < NS; ++s) Fp[(int64_t)flux_comp(NCV + s, full) * Fs] = dsp[s];
+            if (!hyp) {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    double *fp = Fp + (int64_t)flux_comp(c, full) * Fs;
+                    *fp = flux_with_terms(c, kZero, false, dm0, dm1, dm2, de0, de1, de2, de3, c >= NCV ? *fp : 0.0);
                 }
-#pragma unroll
-                for (int c = 0; c < NCV; ++c) fput(c, kZero);
-#pragma unroll
-                for (int s = 0; s < NS; ++s)
-                    fput(NCV + s, T.advect_flag[s] * (kHalf * (velface + fabs(velface)) * pL(FLO + s) +
-                                                      kHalf * (velface - fabs(velface)) * pR(FLO + s)));
-            } else if (!hyp) {
-#pragma unroll
-                for (int c = 0; c < NC; ++c) fput(c, kZero);
             } else {
                 bool first = false;
 #pragma unroll
@@ -497,21 +639,23 @@ __global__ __launch_bounds__(256) void k_faces_lds(StageArgs a, Work w) {
                     double up = N::weno_plus(uc[0], uc[1], uc[2], uc[3], uc[4]);
                     double um = N::weno_minus(uc[1], uc[2], uc[3], uc[4], uc[5]);
                     double h = kHalf * (fp + lam * up) + kHalf * (fm - lam * um);
-                    fput(c, c < NCV ? h : T.advect_flag[c - NCV] * h);
+                    double *fo = Fp + (int64_t)flux_comp(c, full) * Fs;
+                    const double parked = c >= NCV ? *fo : 0.0;
+                    *fo = flux_with_terms(c, c < NCV ? h : T.advect_flag[c - NCV] * h, true, dm0, dm1, dm2, de0, de1, de2, de3,
+                                          parked);
                 }
             }
             if (full) {
 #pragma unroll
                 for (int n = NCV; n < FLO; ++n) F(i, j, k, n) = kZero;
             }
-            face_pass2<P, D>(T, acc, a, i, j, k, F, pL, pR, cL, cR, hyp);
         }
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// Post-step work on one cell (fused epilogue of stage 3):
+// Post-step work on one cell (after the stage-3 combine):
 // Advance_chemistry_implicit (SSPRK3 of AMReX's TimeIntegrator, rk.type=3),
 // update_primitive_vars, clip_neg_speciesconc, residual-norm and dt partials.
 // ---------------------------------------------------------------------------
@@ -613,22 +757,21 @@ MFD void block_reduce_store(double *sums, double m0, double m1, double *out) {
 
 // ---------------------------------------------------------------------------
 // update_residual (div_3d.H:13-76) and the RK3 stage arithmetic
-// (mflo_evolve.cpp:283, 296-298, 309-311) per cell of a tile.
+// (mflo_evolve.cpp:283, 296-298, 309-311) per cell of a tile. Each cell forms
+// its base pointers once (fluxes of the three directions, stage source, U^n,
+// destination), so the loads of all components are independent address
+// offsets from them.
 // ---------------------------------------------------------------------------
-// EPI: stage-3 epilogue; CHEM: chemistry on (then the residual of the conserved
-// components and vfrac is exactly zero: S_new and S_old differ only in the
-// species and, after update_primitive_vars, the primitives).
-template <class P, bool EPI, bool CHEM>
+template <class P>
 __global__ __launch_bounds__(256) void k_residual(ResArgs r, Work w) {
     using N = Num<P>;
     constexpr int NS = P::NS, NV = N::NV;
     const StageArgs &a = r.st;
     if (step_skipped(a.ts)) return;
     const Tab &T = *a.tab;
-    const double time = stage_time(a);
     const double dt = stage_dt(a);
-    (void)time;
     const int nvn = (a.spec_in_solid == 1) ? FLO : FLO + NS;
+    const double dx0 = a.dx[0], dx1 = a.dx[1], dx2 = a.dx[2];
     for (TileIter it(w); it.ok(); it.next()) {
         const int t = it.t;
         int lt;
@@ -638,10 +781,7 @@ __global__ __launch_bounds__(256) void k_residual(ResArgs r, Work w) {
         const int ex = c1[0] - c0[0] + 1, ey = c1[1] - c0[1] + 1, ez = c1[2] - c0[2] + 1;
         const int nc = ex * ey * ez;
         Acc acc = make_acc(a, b);
-        double sums[NV];
-#pragma unroll
-        for (int n = 0; n < NV; ++n) sums[n] = 0.0;
-        double mws = 0.0, mD = 0.0;
+        const View &fx = b.flux[0], &fy = b.flux[1], &fz = b.flux[2];
         for (int f = threadIdx.x; f < nc; f += blockDim.x) {
             const int i = c0[0] + f % ex;
             const int rr = f / ex;
@@ -656,16 +796,35 @@ __global__ __launch_bounds__(256) void k_residual(ResArgs r, Work w) {
 #pragma unroll
                 for (int n = 0; n < FLO; ++n) src[n] = kZero;
             }
-            const double vfc = acc.get(i, j, k, VFRAC, cid);
-            double S[NV], So[NV];
+            const double *px = fx.p + fx.off(i, j, k), *py = fy.p + fy.off(i, j, k), *pz = fz.p + fz.off(i, j, k);
+            const int64_t xs = fx.ns, ys = fy.ns, zs = fz.ns, yj = fy.js, zk = fz.ks;
+            const double *ps = a.src.p + a.src.off(i, j, k);
+            const int64_t sn = a.src.ns;
+            const double *pc = cid >= 0 ? a.cutbuf + cid : ps;   // the values update_cutcells produced
+            const int64_t cn = cid >= 0 ? (int64_t)a.ncut : sn;
+            const double vfc = pc[(int64_t)VFRAC * cn];
+            const double *po = nullptr;
+            int64_t on = 0;
+            if (r.combine && a.stage > 1) {
+                po = r.old.p + r.old.off(i, j, k);
+                on = r.old.ns;
+            }
+            double *pd;
+            int64_t dn;
+            if (r.combine) {
+                pd = r.dst.p + r.dst.off(i, j, k);
+                dn = r.dst.ns;
+            } else {
+                pd = r.dsdt.p + r.dsdt.off(i, j, k);
+                dn = r.dsdt.ns;
+            }
 #pragma unroll
             for (int n = 0; n < NV; ++n) {
                 double d;
                 if (n < NCV || n >= FLO) {
                     const int c = flux_comp(n < NCV ? n : n - (FLO - NCV), a.flux_full);
-                    const View &fx = b.flux[0], &fy = b.flux[1], &fz = b.flux[2];
-                    d = (fx(i, j, k, c) - fx(i + 1, j, k, c)) / a.dx[0] + (fy(i, j, k, c) - fy(i, j + 1, k, c)) / a.dx[1] +
-                        (fz(i, j, k, c) - fz(i, j, k + 1, c)) / a.dx[2] + (n < FLO ? src[n] : kZero);
+                    const double *qx = px + (int64_t)c * xs, *qy = py + (int64_t)c * ys, *qz = pz + (int64_t)c * zs;
+                    d = (qx[0] - qx[1]) / dx0 + (qy[0] - qy[yj]) / dx1 + (qz[0] - qz[zk]) / dx2 + (n < FLO ? src[n] : kZero);
                 } else {
                     d = kZero + src[n];
                 }
@@ -676,73 +835,29 @@ __global__ __launch_bounds__(256) void k_residual(ResArgs r, Work w) {
                     else { if (vfc > kZero && vfc < kOne) d = kZero; }
                 }
                 if (d != d) raise_err(a.err, MF_ERR_NAN_DSDT);
+                double S;
                 if (!r.combine) {
-                    r.dsdt(i, j, k, n) = d;
-                    continue;
-                }
-                if (a.stage == 1) {
-                    S[n] = kOne * acc.get(i, j, k, n, cid) + dt * d;
+                    S = d;
+                } else if (a.stage == 1) {
+                    S = kOne * pc[(int64_t)n * cn] + dt * d;
                 } else if (a.stage == 2) {
-                    double v = a.src(i, j, k, n);
+                    double v = ps[(int64_t)n * sn];
                     v += dt * d;
-                    v += 3.0 * r.old(i, j, k, n);
-                    S[n] = v * kFourth;
+                    v += 3.0 * po[(int64_t)n * on];
+                    S = v * kFourth;
                 } else {
-                    double o = r.old(i, j, k, n);
-                    So[n] = o;
-                    double v = a.src(i, j, k, n);
+                    const double o = po[(int64_t)n * on];
+                    double v = ps[(int64_t)n * sn];
                     v += dt * d;
                     v = o + kTwo * v;
-                    S[n] = v * kThird;
+                    S = v * kThird;
                 }
+                pd[(int64_t)n * dn] = S;
             }
-            if (!r.combine) continue;
-            if (EPI) {
-                if (CHEM) {
-#pragma unroll
-                    for (int n = 0; n < NV; ++n) So[n] = S[n];
-                    chemistry_rk3<P>(T, S, dt);
-                }
-                // update_primitive_vars (mflo_evolve.cpp:129-160)
-                if (S[VFRAC] == kOne) {
-                    double u[N::NC], p[NCV];
-#pragma unroll
-                    for (int c = 0; c < NCV; ++c) u[c] = S[c];
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) u[NCV + s] = S[FLO + s];
-                    N::cons_to_prim(T, u, p);
-#pragma unroll
-                    for (int c = 0; c < NCV; ++c) S[DENS + c] = p[c];
-                    S[TEMP] = P::get_t_from_rpc(T, S[DENS], S[PRES], u + NCV);
-                } else if (a.conj_ht && S[VFRAC] == kZero) {
-                    S[TEMP] = P::get_solid_t_from_rhoe(T, S[RHOE]);
-                }
-                if (r.do_ns && S[PRES] < 0) raise_err(a.err, MF_ERR_NEG_PRESSURE);
-                if (r.clip) {
-#pragma unroll
-                    for (int s = 0; s < NS; ++s) if (S[FLO + s] < 0.0) S[FLO + s] = 0.0;
-                }
-                if (r.track_res) {
-                    const double ia = 1.0 / dt, ib = -1.0 / dt;
-#pragma unroll
-                    for (int n = 0; n < NV; ++n) {
-                        if (CHEM && (n < DENS || n == VFRAC)) continue;
-                        double df = ia * S[n] + ib * So[n];
-                        if (n < VFRAC) df *= So[VFRAC];
-                        sums[n] += df * df;
-                    }
-                }
-                double ws, md;
-                dt_partial<P>(T, S, r.do_ns, ws, md);
-                mws = fmax(mws, ws);
-                mD = fmax(mD, md);
-            }
-#pragma unroll
-            for (int n = 0; n < NV; ++n) r.dst(i, j, k, n) = S[n];
         }
-        if (EPI) block_reduce_store<NV>(sums, mws, mD, r.partials + (size_t)t * (NV + 2));
     }
 }
+
 
 // Post-step pass over the level after the stage-3 combine: S3 is read from
 // st.src (A with chemistry, B without), U^n from `old` (needed by the residual
@@ -757,6 +872,11 @@ __global__ __launch_bounds__(256) void k_post(ResArgs r, Work w) {
     if (step_skipped(a.ts)) return;
     const Tab &T = *a.tab;
     const double dt = stage_dt(a);
+    // partial sums of the block over all its tiles: one slot per block
+    double sums[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) sums[n] = 0.0;
+    double mws = 0.0, mD = 0.0;
     for (TileIter it(w); it.ok(); it.next()) {
         const int t = it.t;
         int lt;
@@ -765,10 +885,6 @@ __global__ __launch_bounds__(256) void k_post(ResArgs r, Work w) {
         tile_range(b, w, lt, c0, c1);
         const int ex = c1[0] - c0[0] + 1, ey = c1[1] - c0[1] + 1, ez = c1[2] - c0[2] + 1;
         const int nc = ex * ey * ez;
-        double sums[NV];
-#pragma unroll
-        for (int n = 0; n < NV; ++n) sums[n] = 0.0;
-        double mws = 0.0, mD = 0.0;
         for (int f = threadIdx.x; f < nc; f += blockDim.x) {
             const int i = c0[0] + f % ex;
             const int rr = f / ex;
@@ -838,8 +954,8 @@ __global__ __launch_bounds__(256) void k_post(ResArgs r, Work w) {
 #pragma unroll
             for (int n = 0; n < NV; ++n) sp[(int64_t)n * sn] = S[n];
         }
-        block_reduce_store<NV>(sums, mws, mD, r.partials + (size_t)t * (NV + 2));
     }
+    block_reduce_store<NV>(sums, mws, mD, r.partials + (size_t)(w.t0 + blockIdx.x) * (NV + 2));
 }
 
 // EstTimeStep partials on the current state (first step of a run)
@@ -848,6 +964,11 @@ __global__ __launch_bounds__(256) void k_est(ResArgs r, Work w) {
     constexpr int NV = Num<P>::NV;
     const StageArgs &a = r.st;
     const Tab &T = *a.tab;
+    // partial sums of the block over all its tiles: one slot per block
+    double sums[NV];
+#pragma unroll
+    for (int n = 0; n < NV; ++n) sums[n] = 0.0;
+    double mws = 0.0, mD = 0.0;
     for (TileIter it(w); it.ok(); it.next()) {
         const int t = it.t;
         int lt;
@@ -855,10 +976,6 @@ __global__ __launch_bounds__(256) void k_est(ResArgs r, Work w) {
         int c0[3], c1[3];
         tile_range(b, w, lt, c0, c1);
         const int ex = c1[0] - c0[0] + 1, ey = c1[1] - c0[1] + 1, ez = c1[2] - c0[2] + 1;
-        double sums[NV];
-#pragma unroll
-        for (int n = 0; n < NV; ++n) sums[n] = 0.0;
-        double mws = 0.0, mD = 0.0;
         for (int f = threadIdx.x; f < ex * ey * ez; f += blockDim.x) {
             const int i = c0[0] + f % ex, rr = f / ex, j = c0[1] + rr % ey, k = c0[2] + rr / ey;
             double S[NV];
@@ -869,8 +986,8 @@ __global__ __launch_bounds__(256) void k_est(ResArgs r, Work w) {
             mws = fmax(mws, ws);
             mD = fmax(mD, md);
         }
-        block_reduce_store<NV>(sums, mws, mD, r.partials + (size_t)t * (NV + 2));
     }
+    block_reduce_store<NV>(sums, mws, mD, r.partials + (size_t)(w.t0 + blockIdx.x) * (NV + 2));
 }
 
 // ---------------------------------------------------------------------------
@@ -1094,11 +1211,19 @@ MFD void cutcell_values(const Tab &T, const CutArgs &a, int i, int j, int k, dou
     for (int c = 0; c < NCV; ++c) out[c] = u[c];
 }
 
+// block -> chunk of the cut list such that each XCD (block b runs on XCD b % 8
+// when the grid is a multiple of 8) walks a contiguous eighth of the list,
+// whose IDW neighbourhoods then share that XCD's L2
+MFD int xcd_chunk(int b, int G) {
+    if (G & 7) return b;
+    return (b & 7) * (G >> 3) + (b >> 3);
+}
+
 template <class P>
 __global__ __launch_bounds__(128) void k_cutcells(CutArgs a) {
     constexpr int NV = Num<P>::NV;
     if (step_skipped(a.ts)) return;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (q >= a.ncut) return;
     double out[NV];
     cutcell_values<P>(*a.tab, a, a.list[q], a.list[a.ncut + q], a.list[2 * a.ncut + q], out);
@@ -1141,7 +1266,9 @@ struct Launch {
     }
     static void cut(const CutArgs &a, hipStream_t st) {
         if (a.ncut <= 0) return;
-        hipLaunchKernelGGL(k_cutcells<P>, dim3((a.ncut + 127) / 128), dim3(128), 0, st, a);
+        int g = (a.ncut + 127) / 128;
+        if (g >= 8) g = (g + 7) & ~7;
+        hipLaunchKernelGGL(k_cutcells<P>, dim3(g), dim3(128), 0, st, a);
     }
     static void cut_box(const CutArgs &a, hipStream_t st) {
         int64_t n = (int64_t)(a.hi[0] - a.lo[0] + 1) * (a.hi[1] - a.lo[1] + 1) * (a.hi[2] - a.lo[2] + 1);
@@ -1165,7 +1292,8 @@ struct Launch {
     static size_t lds_bytes(const int *tsh, int dir) {
         int e[3] = {tsh[0], tsh[1], tsh[2]};
         e[dir] += 6;
-        return (size_t)e[0] * e[1] * e[2] * (Num<P>::NC + 6) * sizeof(double);
+        const size_t ncs = (size_t)e[0] * e[1] * e[2];
+        return ncs * (Num<P>::NC + 6) * sizeof(double) + ((ncs * sizeof(int) + 7) & ~(size_t)7);
     }
     static void faces_lds(const StageArgs &a, const Work &w, int dir, int grid, int block, size_t lds, hipStream_t st) {
         static bool attr = false;
@@ -1180,12 +1308,7 @@ struct Launch {
         else hipLaunchKernelGGL((k_faces_lds<P, 2>), dim3(grid), dim3(block), lds, st, a, w);
     }
     static void residual(const ResArgs &r, const Work &w, int grid, int block, hipStream_t st) {
-        if (r.combine && r.epilogue) {
-            if (r.do_chem) hipLaunchKernelGGL((k_residual<P, true, true>), dim3(grid), dim3(block), 0, st, r, w);
-            else hipLaunchKernelGGL((k_residual<P, true, false>), dim3(grid), dim3(block), 0, st, r, w);
-        } else {
-            hipLaunchKernelGGL((k_residual<P, false, false>), dim3(grid), dim3(block), 0, st, r, w);
-        }
+        hipLaunchKernelGGL(k_residual<P>, dim3(grid), dim3(block), 0, st, r, w);
     }
     static void post(const ResArgs &r, const Work &w, int grid, int block, hipStream_t st) {
         if (r.do_chem) hipLaunchKernelGGL((k_post<P, true>), dim3(grid), dim3(block), 0, st, r, w);

@@ -13,7 +13,9 @@ import numpy as np
 from . import abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmesoflow_amd.so")
+# MESOFLOW_AMD_LIB selects another build of the same library (e.g. one compiled
+# with different tuning macros); default: the in-tree build
+LIB_PATH = os.environ.get("MESOFLOW_AMD_LIB", os.path.join(_HERE, "libmesoflow_amd.so"))
 _LIB = None
 
 # every function of include/mesoflow_amd.h with its argument types
