@@ -405,24 +405,23 @@ MFD double flux_with_terms(int c, double h, bool hyp, double dm0, double dm1, do
     return v;
 }
 // a staged cell seen by a pointwise plugin: conserved components, species and
-// vfrac from the LDS tile, the primitives from memory
+// vfrac from its LDS record, the primitives from memory
 template <int NC>
 struct LdsCell {
-    const double *lds;
-    int ncs, q;
+    const double *rec;
     const Acc *acc;
     int i, j, k, cid;
     MFD double operator()(int n) const {
-        if (n < NCV) return lds[n * ncs + q];
-        if (n >= FLO) return lds[(NCV + n - FLO) * ncs + q];
-        if (n == VFRAC) return lds[(NC + 5) * ncs + q];
+        if (n < NCV) return rec[n];
+        if (n >= FLO) return rec[NCV + n - FLO];
+        if (n == VFRAC) return rec[NC + 5];
         return sel_get(*acc, i, j, k, n, cid);
     }
 };
 
 // minimum waves per SIMD the compiler must keep room for (register budget)
 #ifndef MF_FACE_MIN_WAVES
-#define MF_FACE_MIN_WAVES 3
+#define MF_FACE_MIN_WAVES 2
 #endif
 template <class P, int D>
 __global__ __launch_bounds__(256, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs a, Work w) {
@@ -430,7 +429,9 @@ __global__ __launch_bounds__(256, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
     constexpr int NS = P::NS, NC = N::NC;
     constexpr int e0 = D == 0, e1 = D == 1, e2 = D == 2;
     constexpr int T1 = (D + 1) % 3, T2 = (D + 2) % 3;
-    constexpr int FVX = NC, FP = NC + 3, FG = NC + 4, FVF = NC + 5, NF = NC + 6;
+    // LDS tile: one record of FS doubles per stencil cell (an odd stride keeps
+    // 64-bit reads of consecutive cells on distinct banks)
+    constexpr int FVX = NC, FP = NC + 3, FG = NC + 4, FVF = NC + 5, NF = NC + 6, FS = NF | 1;
     static_assert(P::kGammaIgnoresRhoP, "staged gamma needs a species-only gamma");
     extern __shared__ double lds[];
     if (step_skipped(a.ts)) return;
@@ -448,7 +449,7 @@ __global__ __launch_bounds__(256, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
         const int sx = fx + 5 * e0, sy = fy + 5 * e1, sz = fz + 5 * e2;
         const int ncs = sx * sy * sz;
         const int ox = c0[0] - 3 * e0, oy = c0[1] - 3 * e1, oz = c0[2] - 3 * e2;
-        int *lcid = reinterpret_cast<int *>(lds + NF * ncs);
+        int *lcid = reinterpret_cast<int *>(lds + FS * ncs);
         Acc acc = make_acc(a, b);
         for (int q = threadIdx.x; q < ncs; q += blockDim.x) {
             const int ix = q % sx, r = q / sx, iy = r % sy, iz = r / sy;
@@ -471,16 +472,17 @@ __global__ __launch_bounds__(256, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
                 for (int n = 0; n < NS; ++n) u[NCV + n] = cp[(int64_t)(FLO + n) * cs];
                 vf = cp[(int64_t)VFRAC * cs];
             }
+            double *rec = lds + q * FS;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) lds[c * ncs + q] = u[c];
+            for (int c = 0; c < NC; ++c) rec[c] = u[c];
             double vx, vy, vz, p;
             N::euler_prims(T, u, vx, vy, vz, p);
-            lds[(FVX + 0) * ncs + q] = vx;
-            lds[(FVX + 1) * ncs + q] = vy;
-            lds[(FVX + 2) * ncs + q] = vz;
-            lds[FP * ncs + q] = p;
-            lds[FG * ncs + q] = P::get_gama_from_rpc(T, u[RHO], p, u + NCV);
-            lds[FVF * ncs + q] = vf;
+            rec[FVX + 0] = vx;
+            rec[FVX + 1] = vy;
+            rec[FVX + 2] = vz;
+            rec[FP] = p;
+            rec[FG] = P::get_gama_from_rpc(T, u[RHO], p, u + NCV);
+            rec[FVF] = vf;
             lcid[q] = cid;
         }
         __syncthreads();
@@ -490,11 +492,11 @@ __global__ __launch_bounds__(256, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
             const int ix = f % fx, r = f / fx, iy = r % fy, iz = r / fy;
             const int i = c0[0] + ix, j = c0[1] + iy, k = c0[2] + iz;
             const int q0 = ix + sx * (iy + sy * iz);
-            auto Ld = [&](int field, int m) { return lds[field * ncs + q0 + m * sD]; };
             const int iL = i - e0, jL = j - e1, kL = k - e2;
             const int qL = q0 + 2 * sD, qR = q0 + 3 * sD;
+            const double *recL = lds + qL * FS, *recR = lds + qR * FS;
             const int cL = lcid[qL], cR = lcid[qR];
-            const double vfL = Ld(FVF, 2), vfR = Ld(FVF, 3);
+            const double vfL = recL[FVF], vfR = recR[FVF];
             const double facevfrac = kHalf * (vfL + vfR);
             const bool hyp = facevfrac >= 0.5;
             // ---- viscous stress and work (:293-367), conduction (:373-374), species diffusion (:425-428)
@@ -504,10 +506,10 @@ __global__ __launch_bounds__(256, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
                 {
                     // harmonic mean of (i,j,k) and (i-1,j,k) for every sweep direction (:179-180)
                     double v0, k0, v1, k1;
-                    LdsCell<NC> c0c{lds, ncs, qR, &acc, i, j, k, cR};
+                    LdsCell<NC> c0c{recR, &acc, i, j, k, cR};
                     P::fluid_transport(T, c0c, v0, k0);
                     if (D == 0) {
-                        LdsCell<NC> c1c{lds, ncs, qL, &acc, iL, jL, kL, cL};
+                        LdsCell<NC> c1c{recL, &acc, iL, jL, kL, cL};
                         P::fluid_transport(T, c1c, v1, k1);
                     } else {
                         CellRef c1c{&acc, i - 1, j, k, acc.cid(i - 1, j, k)};
@@ -574,64 +576,63 @@ __global__ __launch_bounds__(256, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
                 const double dTdn = (sel_get(acc, i, j, k, TEMP, cR) - sel_get(acc, iL, jL, kL, TEMP, cL)) / dxs;
                 de3 = thcond * dTdn;
                 double dl[NS], dr[NS];
-                LdsCell<NC> lc{lds, ncs, qL, &acc, iL, jL, kL, cL};
-                LdsCell<NC> rc{lds, ncs, qR, &acc, i, j, k, cR};
+                LdsCell<NC> lc{recL, &acc, iL, jL, kL, cL};
+                LdsCell<NC> rc{recR, &acc, i, j, k, cR};
                 P::spec_dcoeff(T, lc, dl);
                 P::spec_dcoeff(T, rc, dr);
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
                     double Ds = N::harmonic_mean(dl[s], dr[s]);
-                    dsp[s] = Ds * (lds[(NCV + s) * ncs + qR] - lds[(NCV + s) * ncs + qL]) / dxs;
+                    dsp[s] = Ds * (recR[NCV + s] - recL[NCV + s]) / dxs;
                 }
             }
             const View &F = b.flux[D];
             double *Fp = F.p + F.off(i, j, k);
             const int64_t Fs = F.ns;
-            // species diffusion terms parked in the flux slots; the component
-            // loop below subtracts them from the convective flux
-#pragma unroll
-            for (int s = 0; s < NS; ++s) Fp[(int64_t)flux_comp(NCV + s, full) * Fs] = dsp[s];
             if (!hyp) {
 #pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    double *fp = Fp + (int64_t)flux_comp(c, full) * Fs;
-                    *fp = flux_with_terms(c, kZero, false, dm0, dm1, dm2, de0, de1, de2, de3, c >= NCV ? *fp : 0.0);
-                }
+                for (int c = 0; c < NC; ++c)
+                    Fp[(int64_t)flux_comp(c, full) * Fs] =
+                        flux_with_terms(c, kZero, false, dm0, dm1, dm2, de0, de1, de2, de3, c >= NCV ? dsp[c - NCV] : 0.0);
             } else {
                 bool first = false;
 #pragma unroll
-                for (int m = 0; m < 6; ++m) first = first || (Ld(FVF, m) < kOne);
-                const int m0 = first ? 2 : 0, m1 = first ? 2 : 1, m4 = first ? 3 : 4, m5 = first ? 3 : 5;
+                for (int m = 0; m < 6; ++m) first = first || (lds[(q0 + m * sD) * FS + FVF] < kOne);
+                // stencil records, IB first-order fallback (:233-249)
+                const double *rc0 = lds + (first ? qL : q0) * FS;
+                const double *rc1 = lds + (first ? qL : q0 + sD) * FS;
+                const double *rc4 = lds + (first ? qR : q0 + 4 * sD) * FS;
+                const double *rc5 = lds + (first ? qR : q0 + 5 * sD) * FS;
+                const double *rs[6] = {rc0, rc1, recL, recR, rc4, rc5};
                 // max eigenvalue, flux_schemes.H:10-46 (presR with the left species)
                 double lam;
                 {
                     double ul[NC], ur[NC];
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) { ul[c] = Ld(c, 2); ur[c] = Ld(c, 3); }
+                    for (int c = 0; c < NC; ++c) { ul[c] = recL[c]; ur[c] = recR[c]; }
                     double rhoeL = ul[RHOE] - kHalf * (ul[RHOU] * ul[RHOU] + ul[RHOV] * ul[RHOV] + ul[RHOW] * ul[RHOW]) / ul[RHO];
                     double rhoeR = ur[RHOE] - kHalf * (ur[RHOU] * ur[RHOU] + ur[RHOV] * ur[RHOV] + ur[RHOW] * ur[RHOW]) / ur[RHO];
                     double presL = P::get_p_from_rec(T, rhoeL, ul[RHO], ul + NCV);
                     double presR = P::get_p_from_rec(T, rhoeR, ur[RHO], ul + NCV);
-                    double veln = fmax(fabs(Ld(FVX + D, 2)), fabs(Ld(FVX + D, 3)));
-                    double gamaL = Ld(FG, 2), gamaR = Ld(FG, 3);
+                    double veln = fmax(fabs(recL[FVX + D]), fabs(recR[FVX + D]));
+                    double gamaL = recL[FG], gamaR = recR[FG];
                     double ahalf = fmax(sqrt(gamaL * presL / ul[RHO]), sqrt(gamaR * presR / ur[RHO]));
                     lam = ahalf + veln;
                 }
-                const int ms[6] = {m0, m1, 2, 3, m4, m5};
-#pragma unroll 1
+#pragma unroll
                 for (int c = 0; c < NC; ++c) {
                     double uc[6], fc[6];
 #pragma unroll
                     for (int q = 0; q < 6; ++q) {
-                        const int m = ms[q];
-                        uc[q] = Ld(c, m);
-                        const double vn = Ld(FVX + D, m);
+                        const double *rq = rs[q];
+                        uc[q] = rq[c];
+                        const double vn = rq[FVX + D];
                         if (c == RHO) fc[q] = uc[q] * vn;
                         else if (c >= RHOU && c <= RHOW) {
-                            const double va = Ld(FVX + (c - RHOU), m);
-                            const double mm = Ld(RHO, m) * va;
-                            fc[q] = (c - RHOU == D) ? (mm * va + Ld(FP, m)) : (mm * vn);
-                        } else if (c == RHOE) fc[q] = (uc[q] + Ld(FP, m)) * vn;
+                            const double va = rq[FVX + (c - RHOU)];
+                            const double mm = rq[RHO] * va;
+                            fc[q] = (c - RHOU == D) ? (mm * va + rq[FP]) : (mm * vn);
+                        } else if (c == RHOE) fc[q] = (uc[q] + rq[FP]) * vn;
                         else fc[q] = uc[q] * vn;
                     }
                     double fp = N::weno_plus(fc[0], fc[1], fc[2], fc[3], fc[4]);
@@ -639,10 +640,9 @@ __global__ __launch_bounds__(256, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
                     double up = N::weno_plus(uc[0], uc[1], uc[2], uc[3], uc[4]);
                     double um = N::weno_minus(uc[1], uc[2], uc[3], uc[4], uc[5]);
                     double h = kHalf * (fp + lam * up) + kHalf * (fm - lam * um);
-                    double *fo = Fp + (int64_t)flux_comp(c, full) * Fs;
-                    const double parked = c >= NCV ? *fo : 0.0;
-                    *fo = flux_with_terms(c, c < NCV ? h : T.advect_flag[c - NCV] * h, true, dm0, dm1, dm2, de0, de1, de2, de3,
-                                          parked);
+                    Fp[(int64_t)flux_comp(c, full) * Fs] =
+                        flux_with_terms(c, c < NCV ? h : T.advect_flag[c - NCV] * h, true, dm0, dm1, dm2, de0, de1, de2, de3,
+                                        c >= NCV ? dsp[c - NCV] : 0.0);
                 }
             }
             if (full) {
@@ -1293,7 +1293,7 @@ struct Launch {
         int e[3] = {tsh[0], tsh[1], tsh[2]};
         e[dir] += 6;
         const size_t ncs = (size_t)e[0] * e[1] * e[2];
-        return ncs * (Num<P>::NC + 6) * sizeof(double) + ((ncs * sizeof(int) + 7) & ~(size_t)7);
+        return ncs * ((Num<P>::NC + 6) | 1) * sizeof(double) + ((ncs * sizeof(int) + 7) & ~(size_t)7);
     }
     static void faces_lds(const StageArgs &a, const Work &w, int dir, int grid, int block, size_t lds, hipStream_t st) {
         static bool attr = false;
