@@ -189,124 +189,114 @@ struct Rev {
     // for a face with hyp = true. St provides the stencil along D at positions
     // q = 0..5 (offsets -3..+2, with the IB first-order mapping applied):
     //   u(q, c) conserved component c (0..NC-1), vel(q, a), pres(q) the
-    //   geteulerflux primitives; add_u(q, c, x), add_v(q, a, x), add_p(q, x)
-    //   accumulate adjoints of u and of the primitives (the caller turns the
-    //   primitive adjoints into conserved ones with prims_rev per cell).
-    // Positions 2 and 3 are always the true left and right cells.
+    //   geteulerflux primitives; fb(c) the adjoint of flux component c;
+    //   add_u(q, c, x), add_v(q, a, x), add_p(q, x) accumulate adjoints of u
+    //   and of the primitives (the caller turns the primitive adjoints into
+    //   conserved ones with prims_rev per cell).
+    // Positions 2 and 3 are always the true left and right cells. Loop-indexed
+    // values come through St (no register array is indexed by the component).
     // -----------------------------------------------------------------------
     template <int D, class St>
-    MFD static void face_hyp_rev(const Tab &T, St &st, const double *Fb) {
-        double ul[NC], ur[NC];
+    MFD static void face_hyp_rev(const Tab &T, St &st) {
+        double lam;
+        {
+            double ul[NC], ur[NC];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) { ul[c] = st.u(2, c); ur[c] = st.u(3, c); }
-        const double lam = eigen(T, D, ul, ur);
+            for (int c = 0; c < NC; ++c) { ul[c] = st.u(2, c); ur[c] = st.u(3, c); }
+            lam = eigen(T, D, ul, ur);
+        }
         double lamb = 0.0;
-        double vb[6][3], pb[6], rb[6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) { vb[q][0] = vb[q][1] = vb[q][2] = 0.0; pb[q] = 0.0; rb[q] = 0.0; }
 #pragma unroll 1
         for (int c = 0; c < NC; ++c) {
-            const double hb = (c < NCV ? kOne : T.advect_flag[c - NCV]) * Fb[c];
+            const double hb = (c < NCV ? kOne : T.advect_flag[c - NCV]) * st.fb(c);
             if (hb == 0.0) continue;
-            double uc[6], fc[6], ucb[6], fcb[6], r0[6], vn[6];
+            double uc[6], fc[6], g[6];
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
                 uc[q] = st.u(q, c);
-                r0[q] = st.u(q, RHO);
-                vn[q] = st.vel(q, D);
+                const double vn = st.vel(q, D);
                 const double p = st.pres(q);
-                if (c == RHO) fc[q] = uc[q] * vn[q];
+                if (c == RHO) fc[q] = uc[q] * vn;
                 else if (c >= RHOU && c <= RHOW) {
                     const double va = st.vel(q, c - RHOU);
-                    const double mm = r0[q] * va;
-                    fc[q] = (c - RHOU == D) ? (mm * va + p) : (mm * vn[q]);
-                } else if (c == RHOE) fc[q] = (uc[q] + p) * vn[q];
-                else fc[q] = uc[q] * vn[q];
-                ucb[q] = 0.0;
-                fcb[q] = 0.0;
+                    const double mm = st.u(q, RHO) * va;
+                    fc[q] = (c - RHOU == D) ? (mm * va + p) : (mm * vn);
+                } else if (c == RHOE) fc[q] = (uc[q] + p) * vn;
+                else fc[q] = uc[q] * vn;
             }
-            const double up = N::weno_plus(uc[0], uc[1], uc[2], uc[3], uc[4]);
-            const double um = N::weno_minus(uc[1], uc[2], uc[3], uc[4], uc[5]);
-            lamb += kHalf * hb * (up - um);
-            weno_plus_rev(fc, kHalf * hb, fcb);
-            weno_minus_rev(fc + 1, kHalf * hb, fcb + 1);
-            weno_plus_rev(uc, kHalf * lam * hb, ucb);
-            weno_minus_rev(uc + 1, -kHalf * lam * hb, ucb + 1);
+            // reconstructions of u_c: their adjoints go to u_c directly
+            lamb += kHalf * hb * (N::weno_plus(uc[0], uc[1], uc[2], uc[3], uc[4]) - N::weno_minus(uc[1], uc[2], uc[3], uc[4], uc[5]));
+#pragma unroll
+            for (int q = 0; q < 6; ++q) g[q] = 0.0;
+            weno_plus_rev(uc, kHalf * lam * hb, g);
+            weno_minus_rev(uc + 1, -kHalf * lam * hb, g + 1);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) st.add_u(q, c, g[q]);
+            // reconstructions of the flux component: through the cell's flux
+            // into u_c, rho, the velocities and the pressure
+#pragma unroll
+            for (int q = 0; q < 6; ++q) g[q] = 0.0;
+            weno_plus_rev(fc, kHalf * hb, g);
+            weno_minus_rev(fc + 1, kHalf * hb, g + 1);
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const double f = fcb[q];
+                const double f = g[q];
+                const double vn = st.vel(q, D);
                 if (c == RHO) {
-                    ucb[q] += f * vn[q];
-                    vb[q][D] += f * uc[q];
+                    st.add_u(q, c, f * vn);
+                    st.add_v(q, D, f * uc[q]);
                 } else if (c >= RHOU && c <= RHOW) {
                     const int a = c - RHOU;
-                    const double va = st.vel(q, a);
+                    const double va = st.vel(q, a), r0 = st.u(q, RHO);
                     if (a == D) {
-                        rb[q] += f * va * va;
-                        vb[q][a] += f * kTwo * r0[q] * va;
-                        pb[q] += f;
+                        st.add_u(q, RHO, f * va * va);
+                        st.add_v(q, D, f * kTwo * r0 * va);
+                        st.add_p(q, f);
                     } else {
-                        rb[q] += f * va * vn[q];
-                        vb[q][a] += f * r0[q] * vn[q];
-                        vb[q][D] += f * r0[q] * va;
+                        st.add_u(q, RHO, f * va * vn);
+                        st.add_v(q, a, f * r0 * vn);
+                        st.add_v(q, D, f * r0 * va);
                     }
                 } else if (c == RHOE) {
                     const double p = st.pres(q);
-                    ucb[q] += f * vn[q];
-                    pb[q] += f * vn[q];
-                    vb[q][D] += f * (uc[q] + p);
+                    st.add_u(q, c, f * vn);
+                    st.add_p(q, f * vn);
+                    st.add_v(q, D, f * (uc[q] + p));
                 } else {
-                    ucb[q] += f * vn[q];
-                    vb[q][D] += f * uc[q];
+                    st.add_u(q, c, f * vn);
+                    st.add_v(q, D, f * uc[q]);
                 }
-                st.add_u(q, c, ucb[q]);
             }
         }
-        double ubl[NC], ubr[NC];
+        double ul[NC], ur[NC], ubl[NC], ubr[NC];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) { ubl[c] = 0.0; ubr[c] = 0.0; }
+        for (int c = 0; c < NC; ++c) { ul[c] = st.u(2, c); ur[c] = st.u(3, c); ubl[c] = 0.0; ubr[c] = 0.0; }
         eigen_rev(T, D, ul, ur, lamb, ubl, ubr);
 #pragma unroll
         for (int c = 0; c < NC; ++c) { st.add_u(2, c, ubl[c]); st.add_u(3, c, ubr[c]); }
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            st.add_u(q, RHO, rb[q]);
-            st.add_v(q, 0, vb[q][0]);
-            st.add_v(q, 1, vb[q][1]);
-            st.add_v(q, 2, vb[q][2]);
-            st.add_p(q, pb[q]);
-        }
     }
 
     // -----------------------------------------------------------------------
     // Pass 2 of a face (face_pass2<P,D>): viscous stress and work, conduction,
-    // species diffusion. Inputs are the stored primitive components the forward
-    // reads: vL, vR (VELX..VELZ of the left/right cells), tv[8][3] the velocities
-    // of the transverse neighbours in the order L+t1, R+t1, L-t1, R-t1, L+t2,
-    // R+t2, L-t2, R-t2; visc/thcond/Ds the harmonic means. Outputs add to
-    // vLb, vRb, tvb, TLb, TRb, cLb[NS], cRb[NS].
+    // species diffusion. Inputs: vL, vR (stored VELX..VELZ of the left/right
+    // cells), G[q][axis] = d(vel q)/d(axis) as the forward forms it (normal
+    // slot from vR - vL, transverse slots from the averaged neighbours),
+    // visc/thcond/Ds the harmonic means. Outputs add to vLb, vRb (through the
+    // normal derivative and the work term), TLb, TRb, cLb[NS], cRb[NS]; h1[q],
+    // h2[q] are the adjoints of d(vel q) along T1 and T2 divided by 2 dx: the
+    // transverse neighbours get +-h/2 (L+t1, R+t1: +h1/2; L-t1, R-t1: -h1/2;
+    // the same with h2 for t2).
     // -----------------------------------------------------------------------
     template <int D>
     MFD static void pass2_rev(const double *Fb, bool ns, bool hyp, double visc, double thcond, const double *Ds, const double *dx,
-                              const double *vL, const double *vR, const double (*tv)[3], double *vLb, double *vRb,
-                              double (*tvb)[3], double &TLb, double &TRb, double *cLb, double *cRb) {
+                              const double *vL, const double *vR, const double (*G)[3], double *vLb, double *vRb, double *h1,
+                              double *h2, double &TLb, double &TRb, double *cLb, double *cRb) {
         constexpr int T1 = (D + 1) % 3, T2 = (D + 2) % 3;
         const double dxs = dx[D];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) { h1[q] = 0.0; h2[q] = 0.0; }
         if (ns) {
             if (hyp) {
-                // G[q][axis] = d(vel q)/d(axis)
-                double G[3][3];
-#pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    const double gn = (vR[q] - vL[q]) / dxs;
-                    const double top = kHalf * (tv[0][q] + tv[1][q]), bot = kHalf * (tv[2][q] + tv[3][q]);
-                    const double g1 = (top - bot) / (kTwo * dx[T1]);
-                    const double frt = kHalf * (tv[4][q] + tv[5][q]), bck = kHalf * (tv[6][q] + tv[7][q]);
-                    const double g2 = (frt - bck) / (kTwo * dx[T2]);
-                    G[q][D] = gn;
-                    G[q][T1] = g1;
-                    G[q][T2] = g2;
-                }
                 const double delu = G[0][0] + G[1][1] + G[2][2];
                 double t[3];
 #pragma unroll
@@ -342,10 +332,8 @@ struct Rev {
                     const double gn = Gb[q][D] / dxs;
                     vRb[q] += gn;
                     vLb[q] -= gn;
-                    const double h1 = kHalf * Gb[q][T1] / (kTwo * dx[T1]);
-                    tvb[0][q] += h1; tvb[1][q] += h1; tvb[2][q] -= h1; tvb[3][q] -= h1;
-                    const double h2 = kHalf * Gb[q][T2] / (kTwo * dx[T2]);
-                    tvb[4][q] += h2; tvb[5][q] += h2; tvb[6][q] -= h2; tvb[7][q] -= h2;
+                    h1[q] = Gb[q][T1] / (kTwo * dx[T1]);
+                    h2[q] = Gb[q][T2] / (kTwo * dx[T2]);
                 }
             }
             TRb += -Fb[RHOE] * thcond / dxs;
@@ -357,6 +345,21 @@ struct Rev {
             cLb[s] += Fb[NCV + s] * Ds[s] / dxs;
         }
     }
+    // the forward's velocity-gradient tensor of a face from the stored
+    // velocities of L, R and the eight transverse neighbours (order L+t1, R+t1,
+    // L-t1, R-t1, L+t2, R+t2, L-t2, R-t2)
+    template <int D>
+    MFD static void grad_tensor(const double *dx, const double *vL, const double *vR, const double (*tv)[3], double (*G)[3]) {
+        constexpr int T1 = (D + 1) % 3, T2 = (D + 2) % 3;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            G[q][D] = (vR[q] - vL[q]) / dx[D];
+            const double top = kHalf * (tv[0][q] + tv[1][q]), bot = kHalf * (tv[2][q] + tv[3][q]);
+            G[q][T1] = (top - bot) / (kTwo * dx[T1]);
+            const double frt = kHalf * (tv[4][q] + tv[5][q]), bck = kHalf * (tv[6][q] + tv[7][q]);
+            G[q][T2] = (frt - bck) / (kTwo * dx[T2]);
+        }
+    }
 
     // -----------------------------------------------------------------------
     // Chemistry: spec_source of PPorous and the SSP-RK3 of chemistry_rk3
@@ -364,7 +367,10 @@ struct Rev {
     // adjoint of src = spec_source(c, Tm): adds to cb, Tb, logAb
     MFD static void spec_source_rev(const Tab &T, const double *c, double Tm, const double *srcb, double *cb, double &Tb,
                                     double *logAb) {
-        for (int r = 0; r < T.nreact; ++r) {
+        // unrolled over the table size so that logAb[r] stays in registers
+#pragma unroll
+        for (int r = 0; r < MF_MAX_REACTIONS; ++r) {
+            if (r >= T.nreact) break;
             double rateb = 0.0;
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
@@ -536,8 +542,11 @@ struct Rev {
     // the raw neighbourhood values it was computed from. The IDW weights depend
     // on vfrac only.
     // -----------------------------------------------------------------------
+    // fwd (may be null): the cell's forward result, read with stride fstride
+    // instead of recomputing it
     template <class Add>
-    MFD static void cut_rev(const Tab &T, const CutArgs &a, int i, int j, int k, const double *ob, Add &add) {
+    MFD static void cut_rev(const Tab &T, const CutArgs &a, int i, int j, int k, const double *ob, Add &add,
+                            const double *fwd = nullptr, int64_t fstride = 1) {
         const View &s = a.s;
         const double vf = s(i, j, k, VFRAC);
         if (!(vf > kZero && vf < kOne)) {
@@ -545,7 +554,12 @@ struct Rev {
             return;
         }
         double out[NV];
-        cutcell_values<P>(T, a, i, j, k, out);
+        if (fwd) {
+#pragma unroll
+            for (int n = 0; n < NV; ++n) out[n] = fwd[(int64_t)n * fstride];
+        } else {
+            cutcell_values<P>(T, a, i, j, k, out);
+        }
         const double *dx = a.dx;
         double lsg[3] = {0.0, 0.0, 0.0};
         const double zero = 0.0;

@@ -35,21 +35,32 @@ MFK bool dsdt_kept(int n, double vfc, int nvn, int conj_ht) {
     return !(vfc > kZero && vfc < kOne);
 }
 
-// stencil of one face read from the LDS tile (forward values) with LDS adjoint
-// accumulators; positions q = 0..5 map to tile cells q0 + ms[q]*sD
+// LDS tile of the adjoint sweep: one record of AS = 2 NC + 9 doubles (odd) per
+// stencil cell: forward values u[NC], vx, vy, vz, p, vfrac, then the adjoint
+// accumulators ub[NC], vb[3], pb; after the records one slot of FB = NC | 1
+// doubles per thread for the face's flux adjoint
+template <int NC>
+struct AdjRec {
+    static constexpr int FVX = NC, FP = NC + 3, FVF = NC + 4, AU = NC + 5, AV = 2 * NC + 5, AP = 2 * NC + 8;
+    static constexpr int AS = 2 * NC + 9, FB = NC | 1;
+};
+// stencil of one face in the LDS tile; positions q = 0..5 map to records
+// q0 + ms[q]*sD
 template <int NC>
 struct LdsStencil {
+    using R_ = AdjRec<NC>;
     double *lds;
-    int ncs, q0, sD;
+    const double *fbs;
+    int q0, sD;
     int ms[6];
-    static constexpr int FVX = NC, FP = NC + 3, AU = NC + 5, AV = 2 * NC + 5, AP = 2 * NC + 8;
-    MFK int at(int q) const { return q0 + ms[q] * sD; }
-    MFK double u(int q, int c) const { return lds[c * ncs + at(q)]; }
-    MFK double vel(int q, int a) const { return lds[(FVX + a) * ncs + at(q)]; }
-    MFK double pres(int q) const { return lds[FP * ncs + at(q)]; }
-    MFK void add_u(int q, int c, double x) const { if (x != 0.0) atomicAdd(&lds[(AU + c) * ncs + at(q)], x); }
-    MFK void add_v(int q, int a, double x) const { if (x != 0.0) atomicAdd(&lds[(AV + a) * ncs + at(q)], x); }
-    MFK void add_p(int q, double x) const { if (x != 0.0) atomicAdd(&lds[AP * ncs + at(q)], x); }
+    MFK double *rec(int q) const { return lds + (q0 + ms[q] * sD) * R_::AS; }
+    MFK double u(int q, int c) const { return rec(q)[c]; }
+    MFK double vel(int q, int a) const { return rec(q)[R_::FVX + a]; }
+    MFK double pres(int q) const { return rec(q)[R_::FP]; }
+    MFK double fb(int c) const { return fbs[c]; }
+    MFK void add_u(int q, int c, double x) const { if (x != 0.0) atomicAdd(rec(q) + R_::AU + c, x); }
+    MFK void add_v(int q, int a, double x) const { if (x != 0.0) atomicAdd(rec(q) + R_::AV + a, x); }
+    MFK void add_p(int q, double x) const { if (x != 0.0) atomicAdd(rec(q) + R_::AP, x); }
 };
 
 // ---------------------------------------------------------------------------
@@ -69,7 +80,8 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
     constexpr int e0 = D == 0, e1 = D == 1, e2 = D == 2;
     constexpr int T1 = (D + 1) % 3, T2 = (D + 2) % 3;
     using St = LdsStencil<NC>;
-    constexpr int FVX = St::FVX, FP = St::FP, FVF = NC + 4, AU = St::AU, AV = St::AV, AP = St::AP, NT = AP + 1;
+    using RR = AdjRec<NC>;
+    constexpr int FVX = RR::FVX, FP = RR::FP, FVF = RR::FVF, AU = RR::AU, AV = RR::AV, AP = RR::AP, AS = RR::AS;
     extern __shared__ double lds[];
     const StageArgs &a = aa.st;
     if (step_skipped(a.ts)) return;
@@ -88,6 +100,7 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
         const int sx = fx + 5 * e0, sy = fy + 5 * e1, sz = fz + 5 * e2;
         const int ncs = sx * sy * sz;
         const int ox = c0[0] - 3 * e0, oy = c0[1] - 3 * e1, oz = c0[2] - 3 * e2;
+        double *fbs = lds + ncs * AS + threadIdx.x * RR::FB;
         Acc acc = make_acc(a, b);
         for (int q = threadIdx.x; q < ncs; q += blockDim.x) {
             const int ix = q % sx, r = q / sx, iy = r % sy, iz = r / sy;
@@ -98,16 +111,18 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
             for (int n = 0; n < NCV; ++n) u[n] = cp(n);
 #pragma unroll
             for (int n = 0; n < NS; ++n) u[NCV + n] = cp(FLO + n);
+            double *rec = lds + q * AS;
 #pragma unroll
-            for (int c = 0; c < NC; ++c) lds[c * ncs + q] = u[c];
+            for (int c = 0; c < NC; ++c) rec[c] = u[c];
             double vx, vy, vz, p;
             N::euler_prims(T, u, vx, vy, vz, p);
-            lds[(FVX + 0) * ncs + q] = vx;
-            lds[(FVX + 1) * ncs + q] = vy;
-            lds[(FVX + 2) * ncs + q] = vz;
-            lds[FP * ncs + q] = p;
-            lds[FVF * ncs + q] = cp(VFRAC);
-            for (int f = AU; f < NT; ++f) lds[f * ncs + q] = 0.0;
+            rec[FVX + 0] = vx;
+            rec[FVX + 1] = vy;
+            rec[FVX + 2] = vz;
+            rec[FP] = p;
+            rec[FVF] = cp(VFRAC);
+#pragma unroll
+            for (int f = AU; f < AS; ++f) rec[f] = 0.0;
         }
         __syncthreads();
         const int nf = fx * fy * fz;
@@ -119,7 +134,8 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
             const int iL = i - e0, jL = j - e1, kL = k - e2;
             const int fD = D == 0 ? i : (D == 1 ? j : k);
             const bool Rin = fD <= b.hi[D], Lin = fD - 1 >= b.lo[D];
-            const double vfL = lds[FVF * ncs + q0 + 2 * sD], vfR = lds[FVF * ncs + q0 + 3 * sD];
+            double *recL = lds + (q0 + 2 * sD) * AS, *recR = lds + (q0 + 3 * sD) * AS;
+            const double vfL = recL[FVF], vfR = recR[FVF];
             double Fb[NC];
             bool any = false;
 #pragma unroll
@@ -128,6 +144,7 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
                 const double lr = (Rin && dsdt_kept(n, vfR, nvn, a.conj_ht)) ? aa.coef * aa.lam_out(i, j, k, n) : 0.0;
                 const double ll = (Lin && dsdt_kept(n, vfL, nvn, a.conj_ht)) ? aa.coef * aa.lam_out(iL, jL, kL, n) : 0.0;
                 Fb[c] = lr / dxs - ll / dxs;
+                fbs[c] = Fb[c];
                 any = any || Fb[c] != 0.0;
             }
             if (!any) continue;
@@ -136,10 +153,10 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
             if (hyp) {
                 bool first = false;
 #pragma unroll
-                for (int m = 0; m < 6; ++m) first = first || (lds[FVF * ncs + q0 + m * sD] < kOne);
+                for (int m = 0; m < 6; ++m) first = first || (lds[(q0 + m * sD) * AS + FVF] < kOne);
                 St st;
                 st.lds = lds;
-                st.ncs = ncs;
+                st.fbs = fbs;
                 st.q0 = q0;
                 st.sD = sD;
                 st.ms[0] = first ? 2 : 0;
@@ -148,7 +165,7 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
                 st.ms[3] = 3;
                 st.ms[4] = first ? 3 : 4;
                 st.ms[5] = first ? 3 : 5;
-                R::template face_hyp_rev<D>(T, st, Fb);
+                R::template face_hyp_rev<D>(T, st);
             }
             // pass 2: viscous stress and work, conduction, species diffusion
             const int cL = acc.cid(iL, jL, kL), cR = acc.cid(i, j, k);
@@ -176,25 +193,29 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
             const double vL[3] = {pL(VELX), pL(VELY), pL(VELZ)}, vR[3] = {pR(VELX), pR(VELY), pR(VELZ)};
             constexpr int p10 = T1 == 0, p11 = T1 == 1, p12 = T1 == 2;
             constexpr int p20 = T2 == 0, p21 = T2 == 1, p22 = T2 == 2;
+            // transverse neighbours: L+t1, R+t1, L-t1, R-t1, L+t2, R+t2, L-t2, R-t2
             const int ti[8][3] = {{iL + p10, jL + p11, kL + p12}, {i + p10, j + p11, k + p12},
                                   {iL - p10, jL - p11, kL - p12}, {i - p10, j - p11, k - p12},
                                   {iL + p20, jL + p21, kL + p22}, {i + p20, j + p21, k + p22},
                                   {iL - p20, jL - p21, kL - p22}, {i - p20, j - p21, k - p22}};
-            double tv[8][3], tvb[8][3];
             int tc[8];
+            double G[3][3];
 #pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                tc[m] = hyp ? acc.cid(ti[m][0], ti[m][1], ti[m][2]) : -1;
+            for (int q = 0; q < 3; ++q) G[q][0] = G[q][1] = G[q][2] = 0.0;
+            if (hyp) {
+                double tv[8][3];
 #pragma unroll
-                for (int q = 0; q < 3; ++q) {
-                    tv[m][q] = hyp ? acc.get(ti[m][0], ti[m][1], ti[m][2], VELX + q, tc[m]) : 0.0;
-                    tvb[m][q] = 0.0;
+                for (int m = 0; m < 8; ++m) {
+                    tc[m] = acc.cid(ti[m][0], ti[m][1], ti[m][2]);
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) tv[m][q] = acc.get(ti[m][0], ti[m][1], ti[m][2], VELX + q, tc[m]);
                 }
+                R::template grad_tensor<D>(a.dx, vL, vR, tv, G);
             }
-            double vLb[3] = {0.0, 0.0, 0.0}, vRb[3] = {0.0, 0.0, 0.0}, TLb = 0.0, TRb = 0.0, cLb[NS], cRb[NS];
+            double vLb[3] = {0.0, 0.0, 0.0}, vRb[3] = {0.0, 0.0, 0.0}, TLb = 0.0, TRb = 0.0, cLb[NS], cRb[NS], h1[3], h2[3];
 #pragma unroll
             for (int s = 0; s < NS; ++s) { cLb[s] = 0.0; cRb[s] = 0.0; }
-            R::template pass2_rev<D>(Fb, true, hyp, visc, thcond, Ds, a.dx, vL, vR, tv, vLb, vRb, tvb, TLb, TRb, cLb, cRb);
+            R::template pass2_rev<D>(Fb, true, hyp, visc, thcond, Ds, a.dx, vL, vR, G, vLb, vRb, h1, h2, TLb, TRb, cLb, cRb);
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 sink.add(iL, jL, kL, VELX + q, cL, vLb[q]);
@@ -206,12 +227,15 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
 #pragma unroll
                 for (int m = 0; m < 8; ++m)
 #pragma unroll
-                    for (int q = 0; q < 3; ++q) sink.add(ti[m][0], ti[m][1], ti[m][2], VELX + q, tc[m], tvb[m][q]);
+                    for (int q = 0; q < 3; ++q) {
+                        const double h = (m < 4 ? h1[q] : h2[q]) * ((m & 2) ? -kHalf : kHalf);
+                        sink.add(ti[m][0], ti[m][1], ti[m][2], VELX + q, tc[m], h);
+                    }
             }
 #pragma unroll
             for (int s = 0; s < NS; ++s) {
-                if (cLb[s] != 0.0) atomicAdd(&lds[(AU + NCV + s) * ncs + q0 + 2 * sD], cLb[s]);
-                if (cRb[s] != 0.0) atomicAdd(&lds[(AU + NCV + s) * ncs + q0 + 3 * sD], cRb[s]);
+                if (cLb[s] != 0.0) atomicAdd(recL + AU + NCV + s, cLb[s]);
+                if (cRb[s] != 0.0) atomicAdd(recR + AU + NCV + s, cRb[s]);
             }
         }
         __syncthreads();
@@ -219,11 +243,12 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
         for (int q = threadIdx.x; q < ncs; q += blockDim.x) {
             const int ix = q % sx, r = q / sx, iy = r % sy, iz = r / sy;
             const int ci = ox + ix, cj = oy + iy, ck = oz + iz;
+            const double *rec = lds + q * AS;
             double u[NC], ub[NC];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) { u[c] = lds[c * ncs + q]; ub[c] = lds[(AU + c) * ncs + q]; }
-            const double vb[3] = {lds[AV * ncs + q], lds[(AV + 1) * ncs + q], lds[(AV + 2) * ncs + q]};
-            const double pb = lds[AP * ncs + q];
+            for (int c = 0; c < NC; ++c) { u[c] = rec[c]; ub[c] = rec[AU + c]; }
+            const double vb[3] = {rec[AV], rec[AV + 1], rec[AV + 2]};
+            const double pb = rec[AP];
             if (vb[0] != 0.0 || vb[1] != 0.0 || vb[2] != 0.0 || pb != 0.0) R::prims_rev(T, u, vb, pb, ub);
             const int cid = acc.cid(ci, cj, ck);
 #pragma unroll
@@ -248,7 +273,7 @@ __global__ __launch_bounds__(128) void k_cut_adj(AdjCutArgs aa) {
     auto add = [&](int i, int j, int k, int n, double v) {
         if (v != 0.0) unsafeAtomicAdd(&lam(i, j, k, n), v);
     };
-    Rev<P>::cut_rev(*a.tab, a, a.list[q], a.list[a.ncut + q], a.list[2 * a.ncut + q], ob, add);
+    Rev<P>::cut_rev(*a.tab, a, a.list[q], a.list[a.ncut + q], a.list[2 * a.ncut + q], ob, add, a.cutbuf + q, a.ncut);
 }
 
 // adjoint of one phase of k_fill: ghost adjoints go to the cells the ghost was
@@ -383,8 +408,13 @@ __global__ __launch_bounds__(256) void k_post_adj(AdjPostArgs pa, Work w) {
                 for (int n = 0; n < NV; ++n) So[n] = S3[n];
                 R::post_cell(T, So, dt, pa.clip);
                 const double fct = dt * pa.dA * T.molwt[pa.spec];
-                const double c = So[FLO + pa.spec], rho = So[RHO], ru = So[RHOU];
-                lb[FLO + pa.spec] += fct * ru / rho;
+                double c = 0.0;
+#pragma unroll
+                for (int s = 0; s < P::NS; ++s) c = (s == pa.spec) ? So[FLO + s] : c;
+                const double rho = So[RHO], ru = So[RHOU];
+#pragma unroll
+                for (int s = 0; s < P::NS; ++s)
+                    if (s == pa.spec) lb[FLO + s] += fct * ru / rho;
                 lb[RHOU] += fct * c / rho;
                 lb[RHO] += -fct * c * ru / (rho * rho);
             }
@@ -429,7 +459,7 @@ struct AdjLaunch {
     static size_t lds_bytes(const int *tsh, int dir) {
         int e[3] = {tsh[0], tsh[1], tsh[2]};
         e[dir] += 6;
-        return (size_t)e[0] * e[1] * e[2] * (2 * Num<P>::NC + 9) * sizeof(double);
+        return ((size_t)e[0] * e[1] * e[2] * AdjRec<Num<P>::NC>::AS + 128 * AdjRec<Num<P>::NC>::FB) * sizeof(double);
     }
     static void faces(const AdjFaceArgs &a, const Work &w, int dir, int grid, int block, size_t lds, hipStream_t st) {
         static bool attr = false;

@@ -77,7 +77,8 @@ struct Level {
     std::vector<BoxDesc> hbox;
     BoxDesc *d_box = nullptr;
     int ntiles[kTileClasses] = {0, 0, 0, 0, 0, 0, 0};
-    int tsh[kTileClasses][3] = {{64, 4, 1}, {32, 8, 1}, {32, 1, 8}, {64, 2, 2}, {16, 4, 2}, {16, 4, 2}, {16, 2, 4}};
+    // adjoint sweeps (classes 4..6): shapes measured fastest on the 128^3 porous case
+    int tsh[kTileClasses][3] = {{64, 4, 1}, {32, 8, 1}, {32, 1, 8}, {64, 2, 2}, {16, 2, 4}, {8, 16, 1}, {8, 1, 16}};
     int max_grid = 2048;
     double *arr[3] = {nullptr, nullptr, nullptr};  // A, B, C
     size_t padded = 0;

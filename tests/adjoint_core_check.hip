@@ -148,7 +148,9 @@ struct HostStencil {
     int cell[6][3];
     double pr[6][4];                 // vx, vy, vz, p
     std::vector<double> *lam;        // adjoint of the array
+    const double *Fb;                // adjoint of the flux components
     double vb[6][3], pb[6];
+    double fb(int c) const { return Fb[c]; }
     double u(int q, int c) const { return (*s)(cell[q][0], cell[q][1], cell[q][2], c < NCV ? c : FLO + (c - NCV)); }
     double vel(int q, int a) const { return pr[q][a]; }
     double pres(int q) const { return pr[q][3]; }
@@ -201,6 +203,7 @@ static void check_face(const char *tag, HostLevel &L, const Tab &T, int i, int j
     st.T = &T;
     st.s = &s;
     st.lam = &lam;
+    st.Fb = Fb;
     for (int q = 0; q < 6; ++q) {
         const int m = ms[q] - 3;
         st.cell[q][0] = i + m * e0; st.cell[q][1] = j + m * e1; st.cell[q][2] = k + m * e2;
@@ -210,7 +213,7 @@ static void check_face(const char *tag, HostLevel &L, const Tab &T, int i, int j
         st.vb[q][0] = st.vb[q][1] = st.vb[q][2] = 0.0;
         st.pb[q] = 0.0;
     }
-    if (hyp) R::face_hyp_rev<D>(T, st, Fb);
+    if (hyp) R::face_hyp_rev<D>(T, st);
     for (int q = 0; q < 6; ++q) {
         double u[NC], ub[NC];
         for (int c = 0; c < NC; ++c) { u[c] = st.u(q, c); ub[c] = 0.0; }
@@ -240,7 +243,11 @@ static void check_face(const char *tag, HostLevel &L, const Tab &T, int i, int j
         for (int q = 0; q < NS; ++q) Ds[q] = Num<P>::harmonic_mean(dl[q], dr[q]);
     }
     const double visc = Num<P>::harmonic_mean(T.visc, T.visc), thcond = Num<P>::harmonic_mean(T.thcond, T.thcond);
-    R::pass2_rev<D>(Fb, true, hyp, visc, thcond, Ds, dx, vL, vR, tv, vLb, vRb, tvb, TLb, TRb, cLb, cRb);
+    double G[3][3], h1[3], h2[3];
+    R::grad_tensor<D>(dx, vL, vR, tv, G);
+    R::pass2_rev<D>(Fb, true, hyp, visc, thcond, Ds, dx, vL, vR, G, vLb, vRb, h1, h2, TLb, TRb, cLb, cRb);
+    for (int m = 0; m < 8; ++m)
+        for (int q = 0; q < 3; ++q) tvb[m][q] = (m < 4 ? h1[q] : h2[q]) * ((m & 2) ? -0.5 : 0.5);
     auto addc = [&](int a, int b, int c, int n, double x) { lam[s.off(a, b, c) + (size_t)s.ns * n] += x; };
     for (int q = 0; q < 3; ++q) { addc(iL, jL, kL, VELX + q, vLb[q]); addc(i, j, k, VELX + q, vRb[q]); }
     addc(iL, jL, kL, TEMP, TLb);
