@@ -423,28 +423,16 @@ int rebuild_tiles(Level *L) {
 int build_cutlist(Level *L, const double *vfrac_host) {
     const int nx = L->n[0], ny = L->n[1], nz = L->n[2];
     const int64_t ncell = (int64_t)nx * ny * nz;
-    // cut cells ordered by 8^3 bricks (raster order inside a brick), so that a
-    // run of the list covers a compact region and its IDW neighbourhoods overlap
     std::vector<int> id(ncell, -1), li, lj, lk;
-    std::vector<std::pair<int64_t, int64_t>> order;
-    const int B = 8;
-    const int64_t nbx = (nx + B - 1) / B, nby = (ny + B - 1) / B;
+    int nc = 0;
     for (int64_t q = 0; q < ncell; ++q) {
         double v = vfrac_host[q];
         if (v > 0.0 && v < 1.0) {
-            const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((int64_t)nx * ny));
-            const int64_t brick = ((int64_t)(k / B) * nby + j / B) * nbx + i / B;
-            order.push_back({brick, q});
+            id[q] = nc++;
+            li.push_back((int)(q % nx));
+            lj.push_back((int)((q / nx) % ny));
+            lk.push_back((int)(q / ((int64_t)nx * ny)));
         }
-    }
-    std::sort(order.begin(), order.end());
-    int nc = 0;
-    for (const auto &e : order) {
-        const int64_t q = e.second;
-        id[q] = nc++;
-        li.push_back((int)(q % nx));
-        lj.push_back((int)((q / nx) % ny));
-        lk.push_back((int)(q / ((int64_t)nx * ny)));
     }
     if (L->cutlist) { hipFree(L->cutlist); L->cutlist = nullptr; }
     if (L->cutbuf) { hipFree(L->cutbuf); L->cutbuf = nullptr; }
