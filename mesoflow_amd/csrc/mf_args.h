@@ -96,6 +96,12 @@ struct CutArgs {
     double dx[3];
     int spec_in_solid, conj_ht;
     const TimeState *ts;
+    // static geometry of the listed cut cells (vfrac does not change within a
+    // run): geom[4][ncut] = the wall point xnd[0..2] and the signed distance,
+    // mask[4][ncut] = bit (n+2)*25 + (m+2)*5 + (l+2) set for a fluid
+    // (vfrac == 1) neighbour (l, m, n) of the 5^3 neighbourhood
+    const double *geom;
+    const unsigned *mask;
     // dense per-box form (list == nullptr): cells of the box [lo, hi], result
     // to tmp[nv][cells] and flag[cells]
     int lo[3], hi[3];
@@ -155,6 +161,7 @@ struct KernelSet {
     int ns;
     void (*fill)(const FillArgs &, int phase, hipStream_t);
     void (*cut)(const CutArgs &, hipStream_t);
+    void (*cut_geom)(const CutArgs &, double *geom, unsigned *mask, hipStream_t);
     void (*cut_box)(const CutArgs &, hipStream_t);
     void (*faces)(const StageArgs &, const Work &, int dir, int grid, int block, hipStream_t);
     // LDS-staged WENO5 sweeps (order_hyp = 5); lds_bytes(tile) gives the shared memory of a tile

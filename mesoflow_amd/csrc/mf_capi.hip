@@ -87,6 +87,8 @@ struct Level {
     size_t flux_elems = 0;
     int *cutid = nullptr, *cutlist = nullptr;
     double *cutbuf = nullptr;
+    double *cutgeom = nullptr;      // static geometry of the cut cells ([4][ncut])
+    unsigned *cutmask = nullptr;    // fluid-neighbour masks ([4][ncut])
     int ncut = 0;
     double *partials = nullptr;
     size_t partials_cap = 0;
@@ -361,6 +363,8 @@ CutArgs cut_args(Level *L, double *arr) {
     a.list = L->cutlist;
     a.cutbuf = L->cutbuf;
     a.ncut = L->ncut;
+    a.geom = L->cutgeom;
+    a.mask = L->cutmask;
     for (int d = 0; d < 3; ++d) a.dx[d] = c->dx[d];
     a.spec_in_solid = c->prm.species_in_solid;
     a.conj_ht = c->prm.conj_ht;
@@ -434,8 +438,11 @@ int build_cutlist(Level *L, const double *vfrac_host) {
             lk.push_back((int)(q / ((int64_t)nx * ny)));
         }
     }
+    HIPCHK(hipStreamSynchronize(L->c->stream));
     if (L->cutlist) { hipFree(L->cutlist); L->cutlist = nullptr; }
     if (L->cutbuf) { hipFree(L->cutbuf); L->cutbuf = nullptr; }
+    if (L->cutgeom) { hipFree(L->cutgeom); L->cutgeom = nullptr; }
+    if (L->cutmask) { hipFree(L->cutmask); L->cutmask = nullptr; }
     L->ncut = nc;
     HIPCHK(hipMemcpy(L->cutid, id.data(), sizeof(int) * ncell, hipMemcpyHostToDevice));
     if (nc) {
@@ -447,6 +454,15 @@ int build_cutlist(Level *L, const double *vfrac_host) {
         HIPCHK(hipMalloc(&L->cutlist, sizeof(int) * lst.size()));
         HIPCHK(hipMemcpy(L->cutlist, lst.data(), sizeof(int) * lst.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMalloc(&L->cutbuf, sizeof(double) * (size_t)nc * L->c->nv));
+        HIPCHK(hipMalloc(&L->cutgeom, sizeof(double) * 4 * (size_t)nc));
+        HIPCHK(hipMalloc(&L->cutmask, sizeof(unsigned) * 4 * (size_t)nc));
+        // the geometry reads vfrac of the 5^3 neighbourhoods: fill the ghost layers first
+        FillArgs f = fill_args(L, L->arr[0], 1);
+        f.ts = nullptr;
+        f.time_host = 0.0;
+        for (int ph = 0; ph <= 3; ++ph) L->c->ks->fill(f, ph, L->c->stream);
+        L->c->ks->cut_geom(cut_args(L, L->arr[0]), L->cutgeom, L->cutmask, L->c->stream);
+        HIPCHK(hipGetLastError());
     }
     if (L->gexec) { hipGraphExecDestroy(L->gexec); L->gexec = nullptr; }
     if (L->graph) { hipGraphDestroy(L->graph); L->graph = nullptr; }
@@ -1210,6 +1226,8 @@ int mf_level_destroy(void *level) {
     if (L->cutid) hipFree(L->cutid);
     if (L->cutlist) hipFree(L->cutlist);
     if (L->cutbuf) hipFree(L->cutbuf);
+    if (L->cutgeom) hipFree(L->cutgeom);
+    if (L->cutmask) hipFree(L->cutmask);
     if (L->partials) hipFree(L->partials);
     if (L->d_box) hipFree(L->d_box);
     if (L->ts) hipFree(L->ts);

@@ -1219,14 +1219,141 @@ MFD int xcd_chunk(int b, int G) {
     return (b & 7) * (G >> 3) + (b >> 3);
 }
 
+// static geometry of one cut cell (the vfrac-only part of cutcell_values)
+template <class P>
+__global__ __launch_bounds__(128) void k_cut_geom(CutArgs a, double *geom, unsigned *mask) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.ncut) return;
+    const View &s = a.s;
+    const int i = a.list[q], j = a.list[a.ncut + q], k = a.list[2 * a.ncut + q];
+    const double *dx = a.dx;
+    const double vf = s(i, j, k, VFRAC);
+    double lsg[3] = {0.0, 0.0, 0.0};
+    const double zero = 0.0;
+    for (int n = -1; n <= 1; n++)
+        for (int m = -1; m <= 1; m++)
+            for (int l = -1; l <= 1; l++) {
+                double v = s(i + l, j + m, k + n, VFRAC);
+                double bx = ib_basisder1d(l, zero / dx[0]) / dx[0] * ib_basis1d(m, zero / dx[1]) * ib_basis1d(n, zero / dx[2]);
+                double by = ib_basis1d(l, zero / dx[0]) * (ib_basisder1d(m, zero / dx[1]) / dx[1]) * ib_basis1d(n, zero / dx[2]);
+                double bz = ib_basis1d(l, zero / dx[0]) * ib_basis1d(m, zero / dx[1]) * (ib_basisder1d(n, zero / dx[2]) / dx[2]);
+                lsg[0] += v * bx;
+                lsg[1] += v * by;
+                lsg[2] += v * bz;
+            }
+    double mag = sqrt(lsg[0] * lsg[0] + lsg[1] * lsg[1] + lsg[2] * lsg[2]);
+    double dist = kZero, xp[3] = {kZero, kZero, kZero};
+    if (mag > kTiny) {
+        dist = (vf - kHalf) / mag;
+#pragma unroll
+        for (int d = 0; d < 3; d++) xp[d] = -dist * (lsg[d] / mag);
+    }
+    const double dxmin = fmin(dx[0], fmin(dx[1], dx[2]));
+#pragma unroll
+    for (int d = 0; d < 3; ++d) geom[(int64_t)d * a.ncut + q] = xp[d] / dxmin;
+    geom[(int64_t)3 * a.ncut + q] = dist;
+    unsigned mb[4] = {0u, 0u, 0u, 0u};
+    for (int nb = 0; nb < 125; ++nb) {
+        const int l = nb % 5 - 2, m = (nb / 5) % 5 - 2, n = nb / 25 - 2;
+        if (s(i + l, j + m, k + n, VFRAC) == 1.0) mb[nb >> 5] |= 1u << (nb & 31);
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w) mask[(int64_t)w * a.ncut + q] = mb[w];
+}
+
+// update_cutcells of the level's cut-cell list (ib_utils.H:271-436), one thread
+// per cut cell, with the static geometry of k_cut_geom: the IDW sums run over
+// the fluid neighbours of the mask in the reference's loop order.
 template <class P>
 __global__ __launch_bounds__(128) void k_cutcells(CutArgs a) {
-    constexpr int NV = Num<P>::NV;
+    using N = Num<P>;
+    constexpr int NS = P::NS, NV = N::NV;
     if (step_skipped(a.ts)) return;
     const int q = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (q >= a.ncut) return;
+    const Tab &T = *a.tab;
+    const View &s = a.s;
+    const int i = a.list[q], j = a.list[a.ncut + q], k = a.list[2 * a.ncut + q];
     double out[NV];
-    cutcell_values<P>(*a.tab, a, a.list[q], a.list[a.ncut + q], a.list[2 * a.ncut + q], out);
+#pragma unroll
+    for (int n = 0; n < NV; ++n) out[n] = s(i, j, k, n);
+    const double *dx = a.dx;
+    const double dxmin = fmin(dx[0], fmin(dx[1], dx[2]));
+    const double dnd[3] = {dx[0] / dxmin, dx[1] / dxmin, dx[2] / dxmin};
+    const double xnd[3] = {a.geom[q], a.geom[(int64_t)a.ncut + q], a.geom[(int64_t)2 * a.ncut + q]};
+    const double dist = a.geom[(int64_t)3 * a.ncut + q];
+    const double cutoff = dxmin * 1e-5;
+    const bool doT = !a.conj_ht, doS = !a.spec_in_solid;
+    const bool vel0 = fabs(dist) < cutoff;
+    double su0 = 0.0, su1 = 0.0, su2 = 0.0, swv = 0.0;
+    double sp = 0.0, sT = 0.0, sR = 0.0, swn = 0.0, sS[NS];
+#pragma unroll
+    for (int q2 = 0; q2 < NS; ++q2) sS[q2] = 0.0;
+    for (int w = 0; w < 4; ++w) {
+        unsigned bits = a.mask[(int64_t)w * a.ncut + q];
+        while (bits) {
+            const int nb = w * 32 + __ffs(bits) - 1;
+            bits &= bits - 1;
+            const int l = nb % 5 - 2, m = (nb / 5) % 5 - 2, n = nb / 25 - 2;
+            const double *c = s.p + s.off(i + l, j + m, k + n);
+            const int64_t ns = s.ns;
+            if (!vel0 && !(l == 0 && m == 0 && n == 0)) {
+                double a0 = l * dnd[0], a1 = m * dnd[1], a2 = n * dnd[2];
+                double w2 = a0 * a0 + a1 * a1 + a2 * a2 + kTiny2;
+                su0 += c[(int64_t)VELX * ns] / w2;
+                su1 += c[(int64_t)VELY * ns] / w2;
+                su2 += c[(int64_t)VELZ * ns] / w2;
+                swv += 1.0 / w2;
+            }
+            double a0 = l * dnd[0] - xnd[0], a1 = m * dnd[1] - xnd[1], a2 = n * dnd[2] - xnd[2];
+            double w2 = a0 * a0 + a1 * a1 + a2 * a2 + kTiny2;
+            sp += c[(int64_t)PRES * ns] / w2;
+            if (doT) sT += c[(int64_t)TEMP * ns] / w2;
+            sR += c[(int64_t)DENS * ns] / w2;
+            if (doS) {
+#pragma unroll
+                for (int q2 = 0; q2 < NS; ++q2) sS[q2] += c[(int64_t)(FLO + q2) * ns] / w2;
+            }
+            swn += 1.0 / w2;
+        }
+    }
+    // Dirichlet (u = 0) IDW of the velocities, :98-152
+    if (vel0) {
+        out[VELX] = 0.0; out[VELY] = 0.0; out[VELZ] = 0.0;
+    } else {
+        double dn = fabs(dist) / dxmin;
+        double wd = 1.0 / (dn * dn);
+        out[VELX] = (su0 + 0.0 / (dn * dn)) / (swv + wd);
+        out[VELY] = (su1 + 0.0 / (dn * dn)) / (swv + wd);
+        out[VELZ] = (su2 + 0.0 / (dn * dn)) / (swv + wd);
+    }
+    // homogeneous Neumann IDW of p, T, rho (and species without spec_in_solid), :154-198
+    if (swn > 0.0) {
+        out[PRES] = sp / swn;
+        if (doT) out[TEMP] = sT / swn;
+        out[DENS] = sR / swn;
+        if (doS) {
+#pragma unroll
+            for (int q2 = 0; q2 < NS; ++q2) out[FLO + q2] = sS[q2] / swn;
+        }
+    }
+    if (a.conj_ht) {
+        double st = 0.0;
+        int cnt = 0;
+        for (int n = -1; n <= 1; n++)
+            for (int m = -1; m <= 1; m++)
+                for (int l = -1; l <= 1; l++)
+                    if (!(l == 0 && m == 0 && n == 0)) { st += s(i + l, j + m, k + n, TEMP); cnt++; }
+        out[TEMP] = st / (double)cnt;
+    }
+    // prim_to_cons of the updated primitives (:421-433)
+    double u[N::NC], pv[NCV];
+#pragma unroll
+    for (int sp2 = 0; sp2 < NS; ++sp2) u[NCV + sp2] = out[FLO + sp2];
+    pv[0] = out[DENS]; pv[1] = out[VELX]; pv[2] = out[VELY]; pv[3] = out[VELZ]; pv[4] = out[PRES];
+    N::prim_to_cons(T, u, pv);
+#pragma unroll
+    for (int c = 0; c < NCV; ++c) out[c] = u[c];
 #pragma unroll
     for (int n = 0; n < NV; ++n) a.cutbuf[(int64_t)n * a.ncut + q] = out[n];
 }
@@ -1269,6 +1396,10 @@ struct Launch {
         int g = (a.ncut + 127) / 128;
         if (g >= 8) g = (g + 7) & ~7;
         hipLaunchKernelGGL(k_cutcells<P>, dim3(g), dim3(128), 0, st, a);
+    }
+    static void cut_geom(const CutArgs &a, double *geom, unsigned *mask, hipStream_t st) {
+        if (a.ncut <= 0) return;
+        hipLaunchKernelGGL(k_cut_geom<P>, dim3((a.ncut + 127) / 128), dim3(128), 0, st, a, geom, mask);
     }
     static void cut_box(const CutArgs &a, hipStream_t st) {
         int64_t n = (int64_t)(a.hi[0] - a.lo[0] + 1) * (a.hi[1] - a.lo[1] + 1) * (a.hi[2] - a.lo[2] + 1);
@@ -1323,6 +1454,7 @@ struct Launch {
         k.adj = adj;
         k.fill = &fill;
         k.cut = &cut;
+        k.cut_geom = &cut_geom;
         k.cut_box = &cut_box;
         k.faces = &faces;
         k.faces_lds = &faces_lds;
