@@ -155,27 +155,42 @@ struct Rev {
         const double gamaL = P::get_gama_from_rpc(T, ul[RHO], presL, ul + NCV);
         const double gamaR = P::get_gama_from_rpc(T, ur[RHO], presR, ur + NCV);
         const double aL = sqrt(gamaL * presL / ul[RHO]), aR = sqrt(gamaR * presR / ur[RHO]);
-        // veln = fmax(|vL|, |vR|)
-        if (fabs(vL) >= fabs(vR)) {
-            const double vb = lamb * (double)((vL > 0.0) - (vL < 0.0));
-            ubl[RHOU + D] += vb / ul[RHO];
-            ubl[RHO] += -vb * vL / ul[RHO];
-        } else {
-            const double vb = lamb * (double)((vR > 0.0) - (vR < 0.0));
-            ubr[RHOU + D] += vb / ur[RHO];
-            ubr[RHO] += -vb * vR / ur[RHO];
+        // veln = fmax(|vL|, |vR|); the branch the forward took is selected by
+        // value (a select between the two register arrays would put them in
+        // private memory)
+        {
+            const bool left = fabs(vL) >= fabs(vR);
+            const double v = left ? vL : vR, r = left ? ul[RHO] : ur[RHO];
+            const double vb = lamb * (double)((v > 0.0) - (v < 0.0));
+            const double du = vb / r, dr = -vb * v / r;
+            ubl[RHOU + D] += left ? du : 0.0;
+            ubl[RHO] += left ? dr : 0.0;
+            ubr[RHOU + D] += left ? 0.0 : du;
+            ubr[RHO] += left ? 0.0 : dr;
         }
-        double presLb = 0.0, presRb = 0.0;
-        if (aL >= aR) {
-            const double Xb = lamb / (kTwo * aL);
-            gama_rev(T, ul + NCV, Xb * presL / ul[RHO], ubl + NCV);
-            presLb += Xb * gamaL / ul[RHO];
-            ubl[RHO] += -Xb * (gamaL * presL / ul[RHO]) / ul[RHO];
-        } else {
-            const double Xb = lamb / (kTwo * aR);
-            gama_rev(T, ur + NCV, Xb * presR / ur[RHO], ubr + NCV);
-            presRb += Xb * gamaR / ur[RHO];
-            ubr[RHO] += -Xb * (gamaR * presR / ur[RHO]) / ur[RHO];
+        double presLb, presRb;
+        {
+            const bool left = aL >= aR;
+            const double a = left ? aL : aR, r = left ? ul[RHO] : ur[RHO];
+            const double g = left ? gamaL : gamaR, pr = left ? presL : presR;
+            double sp[NG], spb[NG];
+#pragma unroll
+            for (int q = 0; q < NG; ++q) {
+                sp[q] = left ? ul[NCV + q] : ur[NCV + q];
+                spb[q] = 0.0;
+            }
+            const double Xb = lamb / (kTwo * a);
+            gama_rev(T, sp, Xb * pr / r, spb);
+#pragma unroll
+            for (int q = 0; q < NG; ++q) {
+                ubl[NCV + q] += left ? spb[q] : 0.0;
+                ubr[NCV + q] += left ? 0.0 : spb[q];
+            }
+            presLb = left ? Xb * g / r : 0.0;
+            presRb = left ? 0.0 : Xb * g / r;
+            const double rb = -Xb * (g * pr / r) / r;
+            ubl[RHO] += left ? rb : 0.0;
+            ubr[RHO] += left ? 0.0 : rb;
         }
         double rhoeLb = 0.0, rhoeRb = 0.0;
         if (presLb != 0.0) p_rev(T, rhoeL, ul + NCV, presLb, rhoeLb, ubl + NCV);
@@ -206,7 +221,7 @@ struct Rev {
             lam = eigen(T, D, ul, ur);
         }
         double lamb = 0.0;
-#pragma unroll 1
+#pragma unroll
         for (int c = 0; c < NC; ++c) {
             const double hb = (c < NCV ? kOne : T.advect_flag[c - NCV]) * st.fb(c);
             if (hb == 0.0) continue;

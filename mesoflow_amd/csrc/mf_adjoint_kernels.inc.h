@@ -37,15 +37,16 @@ MFK bool dsdt_kept(int n, double vfc, int nvn, int conj_ht) {
 
 // LDS tile of the adjoint sweep: one record of AS = 2 NC + 9 doubles (odd) per
 // stencil cell: forward values u[NC], vx, vy, vz, p, vfrac, then the adjoint
-// accumulators ub[NC], vb[3], pb; after the records one slot of FB = NC | 1
-// doubles per thread for the face's flux adjoint
+// accumulators ub[NC], vb[3], pb
 template <int NC>
 struct AdjRec {
     static constexpr int FVX = NC, FP = NC + 3, FVF = NC + 4, AU = NC + 5, AV = 2 * NC + 5, AP = 2 * NC + 8;
-    static constexpr int AS = 2 * NC + 9, FB = NC | 1;
+    static constexpr int AS = 2 * NC + 9;
 };
 // stencil of one face in the LDS tile; positions q = 0..5 map to records
-// q0 + ms[q]*sD
+// q0 + ms[q]*sD. fbs is the face's flux adjoint, a register array of the
+// calling thread: the reverse sweep indexes it with compile-time components
+// only.
 template <int NC>
 struct LdsStencil {
     using R_ = AdjRec<NC>;
@@ -100,7 +101,6 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
         const int sx = fx + 5 * e0, sy = fy + 5 * e1, sz = fz + 5 * e2;
         const int ncs = sx * sy * sz;
         const int ox = c0[0] - 3 * e0, oy = c0[1] - 3 * e1, oz = c0[2] - 3 * e2;
-        double *fbs = lds + ncs * AS + threadIdx.x * RR::FB;
         Acc acc = make_acc(a, b);
         for (int q = threadIdx.x; q < ncs; q += blockDim.x) {
             const int ix = q % sx, r = q / sx, iy = r % sy, iz = r / sy;
@@ -144,7 +144,6 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
                 const double lr = (Rin && dsdt_kept(n, vfR, nvn, a.conj_ht)) ? aa.coef * aa.lam_out(i, j, k, n) : 0.0;
                 const double ll = (Lin && dsdt_kept(n, vfL, nvn, a.conj_ht)) ? aa.coef * aa.lam_out(iL, jL, kL, n) : 0.0;
                 Fb[c] = lr / dxs - ll / dxs;
-                fbs[c] = Fb[c];
                 any = any || Fb[c] != 0.0;
             }
             if (!any) continue;
@@ -156,7 +155,7 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
                 for (int m = 0; m < 6; ++m) first = first || (lds[(q0 + m * sD) * AS + FVF] < kOne);
                 St st;
                 st.lds = lds;
-                st.fbs = fbs;
+                st.fbs = Fb;
                 st.q0 = q0;
                 st.sD = sD;
                 st.ms[0] = first ? 2 : 0;
@@ -380,7 +379,7 @@ __global__ __launch_bounds__(256) void k_fill_adj(AdjFillArgs aa, int phase) {
 // adjoint of the post-step pass (chemistry, update_primitive_vars, clip) with
 // the outlet term of the objective of this step added to the adjoint of U^{n+1}
 template <class P>
-__global__ __launch_bounds__(256) void k_post_adj(AdjPostArgs pa, Work w) {
+__global__ __launch_bounds__(256, 2) void k_post_adj(AdjPostArgs pa, Work w) {
     constexpr int NV = Num<P>::NV;
     using R = Rev<P>;
     const StageArgs &a = pa.st;
@@ -459,7 +458,7 @@ struct AdjLaunch {
     static size_t lds_bytes(const int *tsh, int dir) {
         int e[3] = {tsh[0], tsh[1], tsh[2]};
         e[dir] += 6;
-        return ((size_t)e[0] * e[1] * e[2] * AdjRec<Num<P>::NC>::AS + 128 * AdjRec<Num<P>::NC>::FB) * sizeof(double);
+        return (size_t)e[0] * e[1] * e[2] * AdjRec<Num<P>::NC>::AS * sizeof(double);
     }
     static void faces(const AdjFaceArgs &a, const Work &w, int dir, int grid, int block, size_t lds, hipStream_t st) {
         static bool attr = false;
