@@ -556,6 +556,35 @@ struct Rev {
     // of the cell's updated values; add(i, j, k, n, x) receives the adjoints of
     // the raw neighbourhood values it was computed from. The IDW weights depend
     // on vfrac only.
+    // adjoint of the cut cell's prim_to_cons and of its primitive outputs: ob
+    // is the adjoint of the updated cell (NV), out its forward values; the
+    // results are the adjoints of the interpolated density, pressure,
+    // temperature, velocities and species
+    MFD static void cut_prim_bars(const Tab &T, const double *out, const double *ob, double &bD, double &bP, double &bT,
+                                  double *bV, double *bS) {
+        const double pv[5] = {out[DENS], out[VELX], out[VELY], out[VELZ], out[PRES]};
+        const double *sp = out + FLO;
+        const double e = P::get_e_from_rpc(T, pv[0], pv[4], sp);
+        double pvb[5] = {ob[RHO], ob[RHOU] * pv[0], ob[RHOV] * pv[0], ob[RHOW] * pv[0], 0.0};
+        double spb[NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) spb[q] = 0.0;
+        pvb[0] += ob[RHOU] * pv[1] + ob[RHOV] * pv[2] + ob[RHOW] * pv[3];
+        pvb[0] += ob[RHOE] * (e + kHalf * (pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3]));
+        pvb[1] += ob[RHOE] * pv[0] * pv[1];
+        pvb[2] += ob[RHOE] * pv[0] * pv[2];
+        pvb[3] += ob[RHOE] * pv[0] * pv[3];
+        e_rev(T, pv[0], pv[4], sp, ob[RHOE] * pv[0], pvb[0], pvb[4], spb);
+        bD = ob[DENS] + pvb[0];
+        bP = ob[PRES] + pvb[4];
+        bT = ob[TEMP];
+        bV[0] = ob[VELX] + pvb[1];
+        bV[1] = ob[VELY] + pvb[2];
+        bV[2] = ob[VELZ] + pvb[3];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) bS[q] = ob[FLO + q] + spb[q];
+    }
+
     // -----------------------------------------------------------------------
     // fwd (may be null): the cell's forward result, read with stride fstride
     // instead of recomputing it
@@ -601,25 +630,8 @@ struct Rev {
         const double xnd[3] = {xp[0] / dxmin, xp[1] / dxmin, xp[2] / dxmin};
         const double cutoff = dxmin * 1e-5;
         const bool doT = !a.conj_ht, doS = !a.spec_in_solid;
-        // prim_to_cons of the updated primitives
-        const double pv[5] = {out[DENS], out[VELX], out[VELY], out[VELZ], out[PRES]};
-        const double *sp = out + FLO;
-        const double e = P::get_e_from_rpc(T, pv[0], pv[4], sp);
-        double pvb[5] = {ob[RHO], ob[RHOU] * pv[0], ob[RHOV] * pv[0], ob[RHOW] * pv[0], 0.0};
-        double spb[NS];
-#pragma unroll
-        for (int q = 0; q < NS; ++q) spb[q] = 0.0;
-        pvb[0] += ob[RHOU] * pv[1] + ob[RHOV] * pv[2] + ob[RHOW] * pv[3];
-        pvb[0] += ob[RHOE] * (e + kHalf * (pv[1] * pv[1] + pv[2] * pv[2] + pv[3] * pv[3]));
-        pvb[1] += ob[RHOE] * pv[0] * pv[1];
-        pvb[2] += ob[RHOE] * pv[0] * pv[2];
-        pvb[3] += ob[RHOE] * pv[0] * pv[3];
-        e_rev(T, pv[0], pv[4], sp, ob[RHOE] * pv[0], pvb[0], pvb[4], spb);
-        const double bD = ob[DENS] + pvb[0], bP = ob[PRES] + pvb[4], bT = ob[TEMP];
-        const double bV[3] = {ob[VELX] + pvb[1], ob[VELY] + pvb[2], ob[VELZ] + pvb[3]};
-        double bS[NS];
-#pragma unroll
-        for (int q = 0; q < NS; ++q) bS[q] = ob[FLO + q] + spb[q];
+        double bD, bP, bT, bV[3], bS[NS];
+        cut_prim_bars(T, out, ob, bD, bP, bT, bV, bS);
         add(i, j, k, VFRAC, ob[VFRAC]);
         if (!doS) {
 #pragma unroll
@@ -689,6 +701,92 @@ struct Rev {
                 for (int m = -1; m <= 1; m++)
                     for (int l = -1; l <= 1; l++)
                         if (!(l == 0 && m == 0 && n == 0)) add(i + l, j + m, k + n, TEMP, bT / 26.0);
+        }
+    }
+
+    // -----------------------------------------------------------------------
+    // cut_rev in gather form (conj_ht = 0). cut_coef, per cut cell with static
+    // geometry g (wall point xnd[3], distance, velocity IDW denominator,
+    // scalar weight sum): the terms the cell's fluid neighbours receive,
+    // divided by the weight sums, as a record of CUTREC doubles
+    //   rec[0..2] velocity adjoints / den (0 when the wall is within the
+    //   cutoff), rec[3] pressure, rec[4] temperature, rec[5] density,
+    //   rec[6..6+NS) species (scalars / sw, 0 without fluid neighbours),
+    //   rec[6+NS..9+NS) the wall point xnd;
+    // the terms of the cell itself go to self(n, v). cut_gather, per fluid
+    // cell: the sum over the cut cells c = p - o of its mask bits o of the
+    // records times the IDW weights of offset o, in the same record order.
+    // -----------------------------------------------------------------------
+    static constexpr int CUTREC = (9 + NS + 1) & ~1;
+    template <class Self>
+    MFD static void cut_coef(const Tab &T, const CutArgs &a, const double *g, const double *ob, const double *out, double *rec,
+                             Self &self) {
+#pragma unroll
+        for (int q = 0; q < CUTREC; ++q) rec[q] = 0.0;
+        const double vf = out[VFRAC];
+        if (!(vf > kZero && vf < kOne)) {
+            for (int n = 0; n < NV; ++n) self(n, ob[n]);
+            return;
+        }
+        const double *dx = a.dx;
+        const double dxmin = fmin(dx[0], fmin(dx[1], dx[2]));
+        const double cutoff = dxmin * 1e-5;
+        const bool doT = !a.conj_ht, doS = !a.spec_in_solid;
+        double bD, bP, bT, bV[3], bS[NS];
+        cut_prim_bars(T, out, ob, bD, bP, bT, bV, bS);
+        self(VFRAC, ob[VFRAC]);
+        if (!doS) {
+#pragma unroll
+            for (int q = 0; q < NS; ++q) self(FLO + q, bS[q]);
+        }
+        if (!(fabs(g[3]) < cutoff)) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) rec[q] = bV[q] / g[4];
+        }
+        const double sw = g[5];
+        if (sw > 0.0) {
+            rec[3] = bP / sw;
+            rec[4] = doT ? bT / sw : 0.0;
+            rec[5] = bD / sw;
+#pragma unroll
+            for (int q = 0; q < NS; ++q) rec[6 + q] = doS ? bS[q] / sw : 0.0;
+        } else {
+            self(PRES, bP);
+            if (doT) self(TEMP, bT);
+            self(DENS, bD);
+            if (doS) {
+#pragma unroll
+                for (int q = 0; q < NS; ++q) self(FLO + q, bS[q]);
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) rec[6 + NS + d] = g[d];
+    }
+    // rec_of(o0, o1, o2) returns the record of the cut cell at offset -o from
+    // the fluid cell; res[0..6+NS) as the record layout
+    template <class RecOf>
+    MFD static void cut_gather(const double *dx, const unsigned *mb, RecOf &rec_of, double *res) {
+        const double dxmin = fmin(dx[0], fmin(dx[1], dx[2]));
+        const double dnd[3] = {dx[0] / dxmin, dx[1] / dxmin, dx[2] / dxmin};
+#pragma unroll
+        for (int q = 0; q < 6 + NS; ++q) res[q] = 0.0;
+        for (int w = 0; w < 4; ++w) {
+            unsigned bits = mb[w];
+            while (bits) {
+                const int b = __builtin_ctz(bits);
+                bits &= bits - 1u;
+                const int nb = w * 32 + b;
+                const int l = nb % 5 - 2, m = (nb / 5) % 5 - 2, n = nb / 25 - 2;
+                const double *r = rec_of(l, m, n);
+                const double a0 = l * dnd[0], a1 = m * dnd[1], a2 = n * dnd[2];
+                const double wv = 1.0 / (a0 * a0 + a1 * a1 + a2 * a2 + kTiny2);
+                const double b0 = a0 - r[6 + NS], b1 = a1 - r[7 + NS], b2 = a2 - r[8 + NS];
+                const double ws = 1.0 / (b0 * b0 + b1 * b1 + b2 * b2 + kTiny2);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) res[q] += r[q] * wv;
+#pragma unroll
+                for (int q = 3; q < 6 + NS; ++q) res[q] += r[q] * ws;
+            }
         }
     }
 

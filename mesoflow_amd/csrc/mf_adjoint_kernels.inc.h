@@ -257,22 +257,68 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
     }
 }
 
-// adjoint of update_cutcells, one thread per cut cell
+// adjoint of update_cutcells in gather form: k_cut_coef, one thread per cut
+// cell, writes the record of terms for the cell's fluid neighbours and adds
+// the cell's own terms; k_cut_gather, one thread per receiving fluid cell,
+// sums the records of the cut cells around it. Every adjoint value has one
+// writer: no atomics.
 template <class P>
-__global__ __launch_bounds__(128) void k_cut_adj(AdjCutArgs aa) {
+__global__ __launch_bounds__(128) void k_cut_coef(AdjCutArgs aa) {
     constexpr int NV = Num<P>::NV;
+    using R = Rev<P>;
     const CutArgs &a = aa.c;
     if (step_skipped(a.ts)) return;
     const int q = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (q >= a.ncut) return;
-    double ob[NV];
+    double ob[NV], out[NV], g[6];
 #pragma unroll
-    for (int n = 0; n < NV; ++n) ob[n] = aa.lcut[(int64_t)n * a.ncut + q];
+    for (int n = 0; n < NV; ++n) {
+        ob[n] = aa.lcut[(int64_t)n * a.ncut + q];
+        out[n] = a.cutbuf[(int64_t)n * a.ncut + q];
+    }
+#pragma unroll
+    for (int d = 0; d < 6; ++d) g[d] = a.geom[(int64_t)d * a.ncut + q];
+    const int i = a.list[q], j = a.list[a.ncut + q], k = a.list[2 * a.ncut + q];
     const View lam = aa.lam;
-    auto add = [&](int i, int j, int k, int n, double v) {
-        if (v != 0.0) unsafeAtomicAdd(&lam(i, j, k, n), v);
+    auto self = [&](int n, double v) {
+        if (v != 0.0) lam(i, j, k, n) += v;
     };
-    Rev<P>::cut_rev(*a.tab, a, a.list[q], a.list[a.ncut + q], a.list[2 * a.ncut + q], ob, add, a.cutbuf + q, a.ncut);
+    double rec[R::CUTREC];
+    R::cut_coef(*a.tab, a, g, ob, out, rec, self);
+    double *dst = aa.coef + (int64_t)q * R::CUTREC;
+#pragma unroll
+    for (int c = 0; c < R::CUTREC; ++c) dst[c] = rec[c];
+}
+
+template <class P>
+__global__ __launch_bounds__(128) void k_cut_gather(AdjCutArgs aa) {
+    constexpr int NS = P::NS;
+    using R = Rev<P>;
+    const CutArgs &a = aa.c;
+    if (step_skipped(a.ts)) return;
+    const int q = xcd_chunk(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (q >= aa.nnbr) return;
+    const int i = aa.nbr[q], j = aa.nbr[aa.nnbr + q], k = aa.nbr[2 * aa.nnbr + q];
+    unsigned mb[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) mb[w] = aa.nbrmask[(int64_t)w * aa.nnbr + q];
+    const int nx = aa.n[0], ny = aa.n[1];
+    auto rec_of = [&](int l, int m, int n) -> const double * {
+        const int cid = aa.cutid[(int64_t)(i - l) + (int64_t)nx * ((int64_t)(j - m) + (int64_t)ny * (k - n))];
+        return aa.coef + (int64_t)cid * R::CUTREC;
+    };
+    double res[6 + NS];
+    R::cut_gather(a.dx, mb, rec_of, res);
+    const View lam = aa.lam;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (res[d] != 0.0) lam(i, j, k, VELX + d) += res[d];
+    if (res[3] != 0.0) lam(i, j, k, PRES) += res[3];
+    if (res[4] != 0.0) lam(i, j, k, TEMP) += res[4];
+    if (res[5] != 0.0) lam(i, j, k, DENS) += res[5];
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+        if (res[6 + s] != 0.0) lam(i, j, k, FLO + s) += res[6 + s];
 }
 
 // adjoint of one phase of k_fill: ghost adjoints go to the cells the ghost was
@@ -472,11 +518,16 @@ struct AdjLaunch {
         else if (dir == 1) hipLaunchKernelGGL((k_faces_adj<P, 1>), dim3(grid), dim3(block), lds, st, a, w);
         else hipLaunchKernelGGL((k_faces_adj<P, 2>), dim3(grid), dim3(block), lds, st, a, w);
     }
+    // grids rounded up to a multiple of 8 blocks for the XCD-chunked mapping
+    static int cut_grid(int n) {
+        int g = (n + 127) / 128;
+        if (g >= 8) g = (g + 7) & ~7;
+        return g;
+    }
     static void cut(const AdjCutArgs &a, hipStream_t st) {
         if (a.c.ncut <= 0) return;
-        int g = (a.c.ncut + 127) / 128;
-        if (g >= 8) g = (g + 7) & ~7;
-        hipLaunchKernelGGL(k_cut_adj<P>, dim3(g), dim3(128), 0, st, a);
+        hipLaunchKernelGGL(k_cut_coef<P>, dim3(cut_grid(a.c.ncut)), dim3(128), 0, st, a);
+        if (a.nnbr > 0) hipLaunchKernelGGL(k_cut_gather<P>, dim3(cut_grid(a.nnbr)), dim3(128), 0, st, a);
     }
     static void fill(const AdjFillArgs &a, int phase, hipStream_t st) {
         const int G = MF_NGROW;
@@ -501,6 +552,7 @@ struct AdjLaunch {
         s.fill = &fill;
         s.post = &post;
         s.objective = &objective;
+        s.cut_rec = Rev<P>::CUTREC;
         return s;
     }
 };

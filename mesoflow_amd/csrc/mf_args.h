@@ -97,7 +97,8 @@ struct CutArgs {
     int spec_in_solid, conj_ht;
     const TimeState *ts;
     // static geometry of the listed cut cells (vfrac does not change within a
-    // run): geom[4][ncut] = the wall point xnd[0..2] and the signed distance,
+    // run): geom[6][ncut] = the wall point xnd[0..2], the signed distance and
+    // the two IDW weight sums (velocity denominator, scalar sum),
     // mask[4][ncut] = bit (n+2)*25 + (m+2)*5 + (l+2) set for a fluid
     // (vfrac == 1) neighbour (l, m, n) of the 5^3 neighbourhood
     const double *geom;
@@ -123,6 +124,15 @@ struct AdjCutArgs {
     CutArgs c;                 // forward stage input array and cut list
     View lam;                  // adjoint of the input array (accumulated)
     const double *lcut;        // adjoint of the cut-cell values
+    double *coef;              // [ncut][record] terms for the fluid neighbours
+    // fluid cells that receive cut-cell terms: nbr[3][nnbr] (i, j, k),
+    // nbrmask[4][nnbr] bit (n+2)*25 + (m+2)*5 + (l+2) set when the cell at
+    // offset -(l, m, n) is a listed cut cell
+    const int *nbr;
+    const unsigned *nbrmask;
+    int nnbr;
+    const int *cutid;          // cut index of valid cell i + nx (j + ny k), or -1
+    int n[3];                  // valid cells of the level
 };
 struct AdjFillArgs {
     FillArgs f;                // forward array (its interior values feed externalbc)
@@ -154,6 +164,7 @@ struct AdjSet {
     void (*fill)(const AdjFillArgs &, int phase, hipStream_t);
     void (*post)(const AdjPostArgs &, const Work &, int grid, int block, hipStream_t);
     void (*objective)(const ObjArgs &, hipStream_t);
+    int cut_rec;               // doubles per cut-cell record of AdjCutArgs::coef
 };
 
 // Host launchers of one compiled plugin family.

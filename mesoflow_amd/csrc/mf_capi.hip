@@ -90,6 +90,7 @@ struct Level {
     double *cutgeom = nullptr;      // static geometry of the cut cells ([4][ncut])
     unsigned *cutmask = nullptr;    // fluid-neighbour masks ([4][ncut])
     int ncut = 0;
+    int geom_gen = 0;  // incremented whenever the cut-cell list and geometry are rebuilt
     double *partials = nullptr;
     size_t partials_cap = 0;
     TimeState *ts = nullptr;
@@ -116,6 +117,12 @@ struct AdjState {
     std::vector<double *> pool;                  // free snapshot buffers
     std::vector<double *> bufs;                  // all snapshot buffers
     std::map<int, double *> snaps;               // step -> snapshot of U^step
+    // gather form of the cut-cell adjoint: per-cut-cell records and the fluid
+    // cells that receive them (rebuilt when the level's geometry changes)
+    double *cutcoef = nullptr;
+    int *nbr = nullptr;
+    unsigned *nbrmask = nullptr;
+    int nnbr = 0, geom_gen = -1;
 };
 
 const KernelSet *pick_kernels(const mf_plugin_params &t) {
@@ -444,6 +451,7 @@ int build_cutlist(Level *L, const double *vfrac_host) {
     if (L->cutgeom) { hipFree(L->cutgeom); L->cutgeom = nullptr; }
     if (L->cutmask) { hipFree(L->cutmask); L->cutmask = nullptr; }
     L->ncut = nc;
+    ++L->geom_gen;
     HIPCHK(hipMemcpy(L->cutid, id.data(), sizeof(int) * ncell, hipMemcpyHostToDevice));
     if (nc) {
         std::vector<int> lst;
@@ -454,7 +462,7 @@ int build_cutlist(Level *L, const double *vfrac_host) {
         HIPCHK(hipMalloc(&L->cutlist, sizeof(int) * lst.size()));
         HIPCHK(hipMemcpy(L->cutlist, lst.data(), sizeof(int) * lst.size(), hipMemcpyHostToDevice));
         HIPCHK(hipMalloc(&L->cutbuf, sizeof(double) * (size_t)nc * L->c->nv));
-        HIPCHK(hipMalloc(&L->cutgeom, sizeof(double) * 4 * (size_t)nc));
+        HIPCHK(hipMalloc(&L->cutgeom, sizeof(double) * 6 * (size_t)nc));
         HIPCHK(hipMalloc(&L->cutmask, sizeof(unsigned) * 4 * (size_t)nc));
         // the geometry reads vfrac of the 5^3 neighbourhoods: fill the ghost layers first
         FillArgs f = fill_args(L, L->arr[0], 1);
@@ -651,14 +659,120 @@ void adj_free(AdjState *A) {
     }
     if (A->grad) hipFree(A->grad);
     if (A->J) hipFree(A->J);
+    if (A->cutcoef) hipFree(A->cutcoef);
+    if (A->nbr) hipFree(A->nbr);
+    if (A->nbrmask) hipFree(A->nbrmask);
     for (double *b : A->bufs) hipFree(b);
     delete A;
+}
+
+// fluid cells that receive cut-cell adjoint terms. k_nbr_flag marks every
+// fluid neighbour of a listed cut cell (its mask bits) in a padded byte array;
+// k_nbr_list, per flagged cell, sets bit (n+2)*25 + (m+2)*5 + (l+2) of its
+// mask for each listed cut cell at offset -(l, m, n), counts the cells
+// (pass 0) or writes them (pass 1, one slot per cell claimed per wave).
+__global__ void k_nbr_flag(const int *list, const unsigned *mask, int ncut, unsigned char *flag, int nx, int ny) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ncut) return;
+    const int G = MF_NGROW;
+    const int64_t PX = nx + 2 * G, PY = ny + 2 * G;
+    const int i = list[q], j = list[ncut + q], k = list[2 * ncut + q];
+    for (int w = 0; w < 4; ++w) {
+        unsigned bits = mask[(int64_t)w * ncut + q];
+        while (bits) {
+            const int nb = w * 32 + __builtin_ctz(bits);
+            bits &= bits - 1u;
+            const int l = nb % 5 - 2, m = (nb / 5) % 5 - 2, n = nb / 25 - 2;
+            flag[(i + l + G) + PX * ((j + m + G) + PY * (int64_t)(k + n + G))] = 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_nbr_list(int pass, const unsigned char *flag, const int *cutid, int nx, int ny, int nz,
+                                                  int *count, int *nbr, unsigned *nbrmask, int cap) {
+    const int G = MF_NGROW;
+    const int PX = nx + 2 * G, PY = ny + 2 * G, PZ = nz + 2 * G;
+    const int64_t npad = (int64_t)PX * PY * PZ;
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    unsigned mb[4] = {0u, 0u, 0u, 0u};
+    int i = 0, j = 0, k = 0;
+    if (p < npad && flag[p]) {
+        i = (int)(p % PX) - G;
+        j = (int)((p / PX) % PY) - G;
+        k = (int)(p / ((int64_t)PX * PY)) - G;
+        for (int nb = 0; nb < 125; ++nb) {
+            const int l = nb % 5 - 2, m = (nb / 5) % 5 - 2, n = nb / 25 - 2;
+            const int ci = i - l, cj = j - m, ck = k - n;
+            if (ci < 0 || ci >= nx || cj < 0 || cj >= ny || ck < 0 || ck >= nz) continue;
+            if (cutid[ci + (int64_t)nx * (cj + (int64_t)ny * ck)] >= 0) mb[nb >> 5] |= 1u << (nb & 31);
+        }
+    }
+    const bool has = (mb[0] | mb[1] | mb[2] | mb[3]) != 0u;
+    const unsigned long long bal = __ballot(has);
+    if (bal == 0ull) return;
+    const int lane = threadIdx.x & 63, lead = __builtin_ctzll(bal);
+    int base = 0;
+    if (lane == lead) base = atomicAdd(count, __popcll(bal));
+    base = __shfl(base, lead, 64);
+    if (pass == 1 && has) {
+        const int slot = base + __popcll(bal & ((1ull << lane) - 1ull));
+        if (slot < cap) {
+            nbr[slot] = i;
+            nbr[cap + slot] = j;
+            nbr[2 * cap + slot] = k;
+            for (int w = 0; w < 4; ++w) nbrmask[(int64_t)w * cap + slot] = mb[w];
+        }
+    }
+}
+
+int adj_build_nbr(Level *L) {
+    Ctx *c = L->c;
+    AdjState *A = L->adj;
+    if (A->nbr) { hipFree(A->nbr); A->nbr = nullptr; }
+    if (A->nbrmask) { hipFree(A->nbrmask); A->nbrmask = nullptr; }
+    A->nnbr = 0;
+    A->geom_gen = L->geom_gen;
+    if (L->ncut == 0) return MF_OK;
+    const int nx = L->n[0], ny = L->n[1], nz = L->n[2], G = MF_NGROW;
+    const int64_t npad = (int64_t)(nx + 2 * G) * (ny + 2 * G) * (nz + 2 * G);
+    unsigned char *flag = nullptr;
+    int *cnt = nullptr;
+    HIPCHK(hipMalloc(&flag, npad));
+    HIPCHK(hipMalloc(&cnt, sizeof(int)));
+    HIPCHK(hipMemsetAsync(flag, 0, npad, c->stream));
+    HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_nbr_flag, dim3((L->ncut + 127) / 128), dim3(128), 0, c->stream, L->cutlist, L->cutmask, L->ncut, flag, nx, ny);
+    const unsigned nblk = (unsigned)((npad + 255) / 256);
+    hipLaunchKernelGGL(k_nbr_list, dim3(nblk), dim3(256), 0, c->stream, 0, flag, L->cutid, nx, ny, nz, cnt, nullptr, nullptr, 0);
+    int n = 0;
+    HIPCHK(hipMemcpyAsync(&n, cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (n > 0) {
+        HIPCHK(hipMalloc(&A->nbr, sizeof(int) * 3 * (size_t)n));
+        HIPCHK(hipMalloc(&A->nbrmask, sizeof(unsigned) * 4 * (size_t)n));
+        HIPCHK(hipMemsetAsync(cnt, 0, sizeof(int), c->stream));
+        hipLaunchKernelGGL(k_nbr_list, dim3(nblk), dim3(256), 0, c->stream, 1, flag, L->cutid, nx, ny, nz, cnt, A->nbr, A->nbrmask, n);
+        int n2 = 0;
+        HIPCHK(hipMemcpyAsync(&n2, cnt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (n2 != n) {
+            hipFree(flag);
+            hipFree(cnt);
+            return MF_ERR_HIP;
+        }
+    }
+    A->nnbr = n;
+    hipFree(flag);
+    hipFree(cnt);
+    HIPCHK(hipGetLastError());
+    return MF_OK;
 }
 
 int adj_alloc(Level *L, int ncheck) {
     Ctx *c = L->c;
     AdjState *A = L->adj;
-    if (A && A->ncheck == ncheck && A->ncut == L->ncut) return MF_OK;
+    if (A && A->ncheck == ncheck && A->ncut == L->ncut)
+        return A->geom_gen == L->geom_gen ? MF_OK : adj_build_nbr(L);
     if (A) { adj_free(A); L->adj = nullptr; }
     A = new (std::nothrow) AdjState();
     if (!A) return MF_ERR_BAD_ARG;
@@ -678,12 +792,13 @@ int adj_alloc(Level *L, int ncheck) {
     }
     HIPCHK(hipMalloc(&A->grad, (MF_MAX_REACTIONS + 1) * sizeof(double)));
     HIPCHK(hipMalloc(&A->J, sizeof(double)));
+    HIPCHK(hipMalloc(&A->cutcoef, (size_t)std::max(L->ncut, 1) * c->ks->adj->cut_rec * sizeof(double)));
     for (int q = 0; q < ncheck; ++q) {
         double *b = nullptr;
         HIPCHK(hipMalloc(&b, pb));
         A->bufs.push_back(b);
     }
-    return MF_OK;
+    return adj_build_nbr(L);
 }
 
 // binomial split of the steps [a, b) with s free snapshots: the first
@@ -803,6 +918,12 @@ void adj_stage_reverse(Level *L, int stage, View lam_out, double coef, double *l
         ca.c.cutbuf = A->cutv[stage - 1];
         ca.lam = lin;
         ca.lcut = A->lcut[stage - 1];
+        ca.coef = A->cutcoef;
+        ca.nbr = A->nbr;
+        ca.nbrmask = A->nbrmask;
+        ca.nnbr = A->nnbr;
+        ca.cutid = L->cutid;
+        for (int d = 0; d < 3; ++d) ca.n[d] = L->n[d];
         S->cut(ca, c->stream);
     }
     AdjFillArgs fl;

@@ -1219,13 +1219,13 @@ MFD int xcd_chunk(int b, int G) {
     return (b & 7) * (G >> 3) + (b >> 3);
 }
 
-// static geometry of one cut cell (the vfrac-only part of cutcell_values)
-template <class P>
-__global__ __launch_bounds__(128) void k_cut_geom(CutArgs a, double *geom, unsigned *mask) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.ncut) return;
+// static geometry of one cut cell (the vfrac-only part of cutcell_values):
+// g[0..2] the wall point / dxmin, g[3] the signed distance, g[4] the
+// denominator of the velocity IDW (fluid-neighbour weights plus the wall
+// term), g[5] the sum of the scalar IDW weights; mb[4] the fluid-neighbour
+// mask, bit (n+2)*25 + (m+2)*5 + (l+2) for neighbour (l, m, n)
+MFD void cut_geom_cell(const CutArgs &a, int i, int j, int k, double *g, unsigned *mb) {
     const View &s = a.s;
-    const int i = a.list[q], j = a.list[a.ncut + q], k = a.list[2 * a.ncut + q];
     const double *dx = a.dx;
     const double vf = s(i, j, k, VFRAC);
     double lsg[3] = {0.0, 0.0, 0.0};
@@ -1249,14 +1249,37 @@ __global__ __launch_bounds__(128) void k_cut_geom(CutArgs a, double *geom, unsig
         for (int d = 0; d < 3; d++) xp[d] = -dist * (lsg[d] / mag);
     }
     const double dxmin = fmin(dx[0], fmin(dx[1], dx[2]));
+    const double dnd[3] = {dx[0] / dxmin, dx[1] / dxmin, dx[2] / dxmin};
 #pragma unroll
-    for (int d = 0; d < 3; ++d) geom[(int64_t)d * a.ncut + q] = xp[d] / dxmin;
-    geom[(int64_t)3 * a.ncut + q] = dist;
-    unsigned mb[4] = {0u, 0u, 0u, 0u};
+    for (int d = 0; d < 3; ++d) g[d] = xp[d] / dxmin;
+    g[3] = dist;
+    mb[0] = mb[1] = mb[2] = mb[3] = 0u;
+    double swv = 0.0, sws = 0.0;
     for (int nb = 0; nb < 125; ++nb) {
         const int l = nb % 5 - 2, m = (nb / 5) % 5 - 2, n = nb / 25 - 2;
-        if (s(i + l, j + m, k + n, VFRAC) == 1.0) mb[nb >> 5] |= 1u << (nb & 31);
+        if (s(i + l, j + m, k + n, VFRAC) == 1.0) {
+            mb[nb >> 5] |= 1u << (nb & 31);
+            const double a0 = l * dnd[0], a1 = m * dnd[1], a2 = n * dnd[2];
+            swv += 1.0 / (a0 * a0 + a1 * a1 + a2 * a2 + kTiny2);
+            const double b0 = a0 - g[0], b1 = a1 - g[1], b2 = a2 - g[2];
+            sws += 1.0 / (b0 * b0 + b1 * b1 + b2 * b2 + kTiny2);
+        }
     }
+    const double dn = fabs(dist) / dxmin;
+    g[4] = swv + 1.0 / (dn * dn);
+    g[5] = sws;
+}
+
+// static geometry of the listed cut cells: geom[6][ncut], mask[4][ncut]
+template <class P>
+__global__ __launch_bounds__(128) void k_cut_geom(CutArgs a, double *geom, unsigned *mask) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.ncut) return;
+    double g[6];
+    unsigned mb[4];
+    cut_geom_cell(a, a.list[q], a.list[a.ncut + q], a.list[2 * a.ncut + q], g, mb);
+#pragma unroll
+    for (int d = 0; d < 6; ++d) geom[(int64_t)d * a.ncut + q] = g[d];
 #pragma unroll
     for (int w = 0; w < 4; ++w) mask[(int64_t)w * a.ncut + q] = mb[w];
 }

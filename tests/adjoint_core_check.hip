@@ -414,6 +414,88 @@ static void check_cut(const Tab &T) {
     if (!tested) { printf("cut: no cut cell found\n"); ++failures; }
 }
 
+// the gather form used by the GPU (cut_geom_cell + cut_coef per cut cell,
+// cut_gather per receiving fluid cell) against cut_rev over every cut cell of
+// a planar wall: the same adjoint up to rounding
+static void check_cut_gather(const Tab &T) {
+    HostLevel L(9, 9, 9);
+    L.fill_random(T);
+    const View &s = L.v;
+    for (int k = -3; k < 12; ++k)
+        for (int j = -3; j < 12; ++j)
+            for (int i = -3; i < 12; ++i) {
+                int fluid = 0;
+                for (int c = 0; c < 8; ++c) {
+                    const double x = i + 0.25 + 0.5 * (c & 1), y = j + 0.25 + 0.5 * ((c >> 1) & 1);
+                    const double z = k + 0.25 + 0.5 * ((c >> 2) & 1);
+                    if (!(0.8 * x + 0.5 * y - 0.35 * z < 3.9)) ++fluid;
+                }
+                s(i, j, k, VFRAC) = fluid / 8.0;
+            }
+    CutArgs a;
+    memset(&a, 0, sizeof(a));
+    a.s = s;
+    a.dx[0] = 4e-6; a.dx[1] = 5e-6; a.dx[2] = 3e-6;
+    std::vector<int> cid(9 * 9 * 9, -1), ci, cj, ck;
+    for (int k = 0; k < 9; ++k)
+        for (int j = 0; j < 9; ++j)
+            for (int i = 0; i < 9; ++i) {
+                const double vf = s(i, j, k, VFRAC);
+                if (vf > 0.0 && vf < 1.0) {
+                    cid[i + 9 * (j + 9 * k)] = (int)ci.size();
+                    ci.push_back(i); cj.push_back(j); ck.push_back(k);
+                }
+            }
+    const int ncut = (int)ci.size();
+    std::vector<double> lref(L.a.size(), 0.0), lnew(L.a.size(), 0.0), recs((size_t)ncut * R::CUTREC);
+    auto at = [&](std::vector<double> &v, int i, int j, int k, int n) -> double & { return v[s.off(i, j, k) + (size_t)s.ns * n]; };
+    for (int q = 0; q < ncut; ++q) {
+        double ob[NV], out[NV], g[6];
+        unsigned mb[4];
+        for (int n = 0; n < NV; ++n) ob[n] = urand(-1, 1);
+        cutcell_values<P>(T, a, ci[q], cj[q], ck[q], out);
+        auto add = [&](int i_, int j_, int k_, int n, double x) { at(lref, i_, j_, k_, n) += x; };
+        R::cut_rev(T, a, ci[q], cj[q], ck[q], ob, add, out, 1);
+        cut_geom_cell(a, ci[q], cj[q], ck[q], g, mb);
+        auto self = [&](int n, double x) { at(lnew, ci[q], cj[q], ck[q], n) += x; };
+        R::cut_coef(T, a, g, ob, out, &recs[(size_t)q * R::CUTREC], self);
+    }
+    int receivers = 0;
+    for (int k = -2; k < 11; ++k)
+        for (int j = -2; j < 11; ++j)
+            for (int i = -2; i < 11; ++i) {
+                if (s(i, j, k, VFRAC) != 1.0) continue;
+                unsigned mb[4] = {0u, 0u, 0u, 0u};
+                for (int nb = 0; nb < 125; ++nb) {
+                    const int l = nb % 5 - 2, m = (nb / 5) % 5 - 2, n = nb / 25 - 2;
+                    const int x = i - l, y = j - m, z = k - n;
+                    if (x < 0 || x >= 9 || y < 0 || y >= 9 || z < 0 || z >= 9) continue;
+                    if (cid[x + 9 * (y + 9 * z)] >= 0) mb[nb >> 5] |= 1u << (nb & 31);
+                }
+                if (!(mb[0] | mb[1] | mb[2] | mb[3])) continue;
+                ++receivers;
+                auto rec_of = [&](int l, int m, int n) -> const double * {
+                    return &recs[(size_t)cid[(i - l) + 9 * ((j - m) + 9 * (k - n))] * R::CUTREC];
+                };
+                double res[6 + NS];
+                R::cut_gather(a.dx, mb, rec_of, res);
+                for (int d = 0; d < 3; ++d) at(lnew, i, j, k, VELX + d) += res[d];
+                at(lnew, i, j, k, PRES) += res[3];
+                at(lnew, i, j, k, TEMP) += res[4];
+                at(lnew, i, j, k, DENS) += res[5];
+                for (int q = 0; q < NS; ++q) at(lnew, i, j, k, FLO + q) += res[6 + q];
+            }
+    double dmax = 0.0, rmax = 0.0;
+    for (size_t q = 0; q < lref.size(); ++q) {
+        dmax = fmax(dmax, fabs(lnew[q] - lref[q]));
+        rmax = fmax(rmax, fabs(lref[q]));
+    }
+    const bool ok = ncut > 20 && receivers > 20 && dmax <= 1e-12 * rmax;
+    printf("%-34s %d cut cells %d receivers  max|diff| %.3e  max|adjoint| %.3e %s\n", "cut cell gather form", ncut, receivers,
+           dmax, rmax, ok ? "ok" : "FAIL");
+    if (!ok) ++failures;
+}
+
 int main() {
     const Tab T = make_tab();
     check_weno();
@@ -438,6 +520,7 @@ int main() {
     check_post(T);
     check_externalbc(T);
     check_cut(T);
+    check_cut_gather(T);
     printf("%s: %d failure(s)\n", failures ? "FAILED" : "PASSED", failures);
     return failures ? 1 : 0;
 }

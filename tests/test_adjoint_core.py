@@ -29,5 +29,6 @@ def test_adjoint_reverse_functions_match_finite_differences(tmp_path):
     assert r.returncode == 0 and "PASSED" in r.stdout, r.stdout[-4000:]
     # every family of checks ran
     for name in ("weno_plus", "weno_minus", "face D=0", "face D=1", "face D=2", "IB first order", "no hyperbolic part",
-                 "post state", "post log_A", "externalbc inflow in_vel", "externalbc outflow", "cut cell"):
+                 "post state", "post log_A", "externalbc inflow in_vel", "externalbc outflow", "cut cell",
+                 "cut cell gather form"):
         assert name in r.stdout, name
