@@ -228,6 +228,32 @@ int mf_level_step(void *level, int nsteps, int mode);
  * build; no counterpart in the reference. */
 int mf_level_step_timed(void *level, int nsteps, int mode, double *class_ms, int *class_launches);
 
+/* EvolveAMR(final_time, only_flow) (Source/mflo_evolve.cpp:18-89): steps while
+ * istep < max_step and t < final_time (ComputeDt per step, the stop_time
+ * break); only_flow = 1 leaves out the chemistry of each step
+ * (Advance_coupled_strang :314-317). *nsteps_out (may be NULL) receives the
+ * number of steps taken. */
+int mf_level_evolve(void *level, double final_time, int max_step, int only_flow, int mode, int *nsteps_out);
+
+/* Chemistry phase of the split mode (Source/mflo_splitchem.cpp:40-62):
+ * Advance_chemistry (:172-195, explicit midpoint) substeps of dt_react_rk from
+ * react_time while tchem < dt_react, then update_vars_after_chemsolve
+ * (:74-140). The first stage of a substep reads the reference's other state
+ * buffer (phi_old after its swap): the state before the previous flow step or
+ * substep; after mf_level_set_state, the state itself. Returns
+ * MF_ERR_UNSUPPORTED after coupled steps (mf_level_step, the adjoint), whose
+ * pre-chemistry state is not kept, and with using_bg_inertgas = 1 (no plugin
+ * here has a background gas). All substeps of a cell run in one thread. */
+int mf_level_advance_chemistry(void *level, double react_time, double dt_react, double dt_react_rk);
+
+/* Evolve_split (Source/mflo_splitchem.cpp:24-72; main.cpp:36-56 reads its
+ * five times from mflo.steady_flow_time, react_final_time,
+ * react_increment_time, react_time_step, flow_coupling_time): flow-only
+ * steps to t_ss, then until react_time >= t_react a chemistry phase of
+ * dt_react followed by flow-only steps to the next coupling time (+dt_ss). */
+int mf_level_evolve_split(void *level, double t_ss, double t_react, double dt_react, double dt_react_rk, double dt_ss,
+                          int max_step, int mode);
+
 /* Time, last dt, step count and residual norms (TOTAL_NVARS doubles, may be NULL). */
 int mf_level_info(void *level, double *time, double *dt, int *istep, double *resnorms);
 /* Overrides the level time/dt history (restart). */

@@ -167,6 +167,19 @@ struct AdjSet {
     int cut_rec;               // doubles per cut-cell record of AdjCutArgs::coef
 };
 
+// chemistry phase of the split mode over the valid cells of a level
+struct ChemSplitArgs {
+    const Tab *tab;
+    View s;                    // current state (phi_new), updated in place
+    View lag;                  // the other buffer (phi_old): first-stage state of the next substep
+    int n[3];                  // valid cells
+    int nsub;                  // substeps of this launch
+    double dt;                 // react_time_step
+    int finish;                // 1: update_vars_after_chemsolve after the substeps
+    int do_ns;
+    int *err;
+};
+
 // Host launchers of one compiled plugin family.
 struct KernelSet {
     int ns;
@@ -181,6 +194,7 @@ struct KernelSet {
     void (*residual)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
     void (*est)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
     void (*post)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
+    void (*chem_split)(const ChemSplitArgs &, hipStream_t);
     const AdjSet *adj;         // discrete adjoint (null: not compiled for this family)
 };
 

@@ -95,6 +95,14 @@ struct Level {
     size_t partials_cap = 0;
     TimeState *ts = nullptr;
     bool need_est = true;
+    // split mode (mf_level_evolve / mf_level_advance_chemistry): only_flow of
+    // the steps being enqueued, the graph's value of it, and the other state
+    // buffer of the reference (phi_old) that the chemistry phase's first
+    // stage reads: lag_mode 0 unknown (after a coupled step), 1 equal to the
+    // current state (after set_state), 2 held in lag
+    int only_flow = 0, graph_only_flow = 0;
+    double *lag = nullptr;
+    int lag_mode = 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     struct AdjState *adj = nullptr;
@@ -549,8 +557,9 @@ void enqueue_stage(Level *L, int stage, int mode) {
     r.old = L->view[0];
     // stage 3 writes A in place with chemistry; without, B keeps S3 so that the
     // post pass still sees U^n in A for the residual
-    r.dst = L->view[stage == 1 ? 1 : (stage == 2 ? 2 : (c->prm.do_chemistry ? 0 : 1))];
-    r.do_chem = c->prm.do_chemistry;
+    const int chem = c->prm.do_chemistry && !L->only_flow;
+    r.dst = L->view[stage == 1 ? 1 : (stage == 2 ? 2 : (chem ? 0 : 1))];
+    r.do_chem = chem;
     r.clip = c->prm.clip_species;
     r.track_res = c->prm.track_residual_norms;
     r.do_ns = c->prm.do_ns;
@@ -1352,6 +1361,7 @@ int mf_level_destroy(void *level) {
     if (L->partials) hipFree(L->partials);
     if (L->d_box) hipFree(L->d_box);
     if (L->ts) hipFree(L->ts);
+    if (L->lag) hipFree(L->lag);
     adj_free(L->adj);
     delete L;
     return MF_OK;
@@ -1379,6 +1389,7 @@ int mf_level_set_state(void *level, const double *state, int is_device) {
     int st = build_cutlist(L, vf.data());
     if (st) return st;
     L->need_est = true;
+    L->lag_mode = 1;
     return MF_OK;
 }
 
@@ -1451,16 +1462,23 @@ int mf_est_dt(void *level, double *dt_out) {
     return MF_OK;
 }
 
-int mf_level_step(void *level, int nsteps, int mode) {
-    Level *L = (Level *)level;
-    if (!L || nsteps < 0 || mode < 0 || mode > 2) return MF_ERR_BAD_ARG;
+// nsteps steps of the level (one graph replay per step in graph mode)
+int level_steps(Level *L, int nsteps, int mode) {
     Ctx *c = L->c;
     if (L->need_est) {
         enqueue_est(L);
         L->need_est = false;
     }
     if (mode == MF_MODE_GRAPH) {
+        if (L->gexec && L->graph_only_flow != L->only_flow) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            hipGraphExecDestroy(L->gexec);
+            hipGraphDestroy(L->graph);
+            L->gexec = nullptr;
+            L->graph = nullptr;
+        }
         if (!L->gexec) {
+            L->graph_only_flow = L->only_flow;
             hipStream_t cap = c->stream;
             bool own = false;
             if (!cap) {
@@ -1483,8 +1501,99 @@ int mf_level_step(void *level, int nsteps, int mode) {
         for (int s = 0; s < nsteps; ++s) enqueue_step(L, mode);
     }
     HIPCHK(hipGetLastError());
+    return MF_OK;
+}
+
+int mf_level_step(void *level, int nsteps, int mode) {
+    Level *L = (Level *)level;
+    if (!L || nsteps < 0 || mode < 0 || mode > 2) return MF_ERR_BAD_ARG;
+    int st = level_steps(L, nsteps, mode);
+    if (st) return st;
+    if (nsteps > 0) L->lag_mode = 0;
     TimeState h;
-    int st = read_ts(L, &h);
+    st = read_ts(L, &h);
+    if (st) return st;
+    return h.err;
+}
+
+// ---------------------------------------------------------------------------
+// split-chemistry mode (Source/mflo_splitchem.cpp; EvolveAMR with only_flow)
+// ---------------------------------------------------------------------------
+int lag_alloc(Level *L) {
+    if (!L->lag) HIPCHK(hipMalloc(&L->lag, L->padded * sizeof(double)));
+    return MF_OK;
+}
+
+int level_evolve(Level *L, double final_time, int max_step, int only_flow, int mode, int *taken) {
+    Ctx *c = L->c;
+    int n = 0, st = MF_OK;
+    TimeState h;
+    st = read_ts(L, &h);
+    if (st) return st;
+    if (h.err) return h.err;
+    if (only_flow && (st = lag_alloc(L))) return st;
+    L->only_flow = only_flow;
+    while (h.istep < max_step && h.t < final_time && h.active) {
+        // the reference's phi_old after a flow-only step is the state the step started from
+        if (only_flow) {
+            const int64_t np = (int64_t)L->padded;
+            hipLaunchKernelGGL(k_copy, dim3((unsigned)std::min<int64_t>((np + 255) / 256, 65536)), dim3(256), 0, c->stream, L->lag,
+                               L->arr[0], np);
+        }
+        st = level_steps(L, 1, mode);
+        if (st) break;
+        ++n;
+        st = read_ts(L, &h);
+        if (st) break;
+        if (h.err) { st = h.err; break; }
+    }
+    L->only_flow = 0;
+    if (n > 0) L->lag_mode = only_flow ? 2 : 0;
+    if (taken) *taken = n;
+    return st;
+}
+
+int level_chem_phase(Level *L, double react_time, double dt_react, double dt_rk) {
+    (void)react_time;
+    Ctx *c = L->c;
+    if (c->prm.using_bg_inertgas) return MF_ERR_UNSUPPORTED;
+    if (!(dt_rk > 0.0) || !(dt_react < 1e300)) return MF_ERR_BAD_ARG;
+    if (L->lag_mode == 0) return MF_ERR_UNSUPPORTED;
+    int st = lag_alloc(L);
+    if (st) return st;
+    if (L->lag_mode == 1) {
+        const int64_t np = (int64_t)L->padded;
+        hipLaunchKernelGGL(k_copy, dim3((unsigned)std::min<int64_t>((np + 255) / 256, 65536)), dim3(256), 0, c->stream, L->lag,
+                           L->arr[0], np);
+    }
+    // the substeps of `while (tchem < dt_react) { ...; tchem += dt_rk; }`
+    int64_t nsub = 0;
+    for (double tchem = 0.0; tchem < dt_react; tchem += dt_rk) {
+        if (++nsub > (int64_t)1 << 40) return MF_ERR_BAD_ARG;
+    }
+    ChemSplitArgs a;
+    a.tab = c->d_tab;
+    a.s = L->view[0];
+    a.lag = padded_view(L->lag, L->n, c->nv);
+    for (int d = 0; d < 3; ++d) a.n[d] = L->n[d];
+    a.dt = dt_rk;
+    a.do_ns = c->prm.do_ns;
+    a.err = c->d_err;
+    // launches of at most kChunk substeps (the kernel continues the sequence exactly)
+    const int64_t kChunk = 1024;
+    int64_t done = 0;
+    do {
+        const int64_t m = std::min(kChunk, nsub - done);
+        a.nsub = (int)m;
+        done += m;
+        a.finish = done == nsub;
+        c->ks->chem_split(a, c->stream);
+    } while (done < nsub);
+    HIPCHK(hipGetLastError());
+    L->lag_mode = 2;
+    L->need_est = true;  // the next ComputeDt sees the state after the chemistry
+    TimeState h;
+    st = read_ts(L, &h);
     if (st) return st;
     return h.err;
 }
@@ -1519,6 +1628,38 @@ int mf_level_step_timed(void *level, int nsteps, int mode, double *class_ms, int
     L->timer.ev.clear();
     if (st) return st;
     return h.err;
+}
+
+int mf_level_evolve(void *level, double final_time, int max_step, int only_flow, int mode, int *nsteps_out) {
+    Level *L = (Level *)level;
+    if (!L || mode < 0 || mode > 2 || max_step < 0) return MF_ERR_BAD_ARG;
+    return level_evolve(L, final_time, max_step, only_flow ? 1 : 0, mode, nsteps_out);
+}
+
+int mf_level_advance_chemistry(void *level, double react_time, double dt_react, double dt_react_rk) {
+    Level *L = (Level *)level;
+    if (!L) return MF_ERR_BAD_ARG;
+    return level_chem_phase(L, react_time, dt_react, dt_react_rk);
+}
+
+int mf_level_evolve_split(void *level, double t_ss, double t_react, double dt_react, double dt_react_rk, double dt_ss,
+                          int max_step, int mode) {
+    Level *L = (Level *)level;
+    if (!L || mode < 0 || mode > 2 || max_step < 0) return MF_ERR_BAD_ARG;
+    if (L->c->prm.using_bg_inertgas) return MF_ERR_UNSUPPORTED;
+    double react_time = 0.0, flow_time;
+    int st = level_evolve(L, t_ss, max_step, 1, mode, nullptr);
+    if (st) return st;
+    flow_time = t_ss;
+    while (react_time < t_react) {
+        st = level_chem_phase(L, react_time, dt_react, dt_react_rk);
+        if (st) return st;
+        st = level_evolve(L, flow_time + dt_ss, max_step, 1, mode, nullptr);
+        if (st) return st;
+        react_time += dt_react;
+        flow_time += dt_ss;
+    }
+    return MF_OK;
 }
 
 int mf_level_info(void *level, double *time, double *dt, int *istep, double *resnorms) {

@@ -104,6 +104,8 @@ struct Tab : mf_plugin_params {
 struct PShockTubeN2 {
     static constexpr int NS = 1;
     static constexpr int NG = 1;
+    // thermo.H:39-48
+    MFD static double get_r_from_c(const Tab &T, const double *spec) { return kZero + spec[0] * T.molwt[0]; }
     MFD static double get_t_from_rpc(const Tab &T, double rho, double pres, const double *spec) {
         double sum_ci = kZero + spec[0];
         return pres / (sum_ci * T.R_univ);

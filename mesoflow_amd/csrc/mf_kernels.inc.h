@@ -702,6 +702,17 @@ MFD void chemistry_rk3(const Tab &T, double *S, double dt) {
     }
 }
 
+// a x + b y rounded as written: the empty asm makes both products opaque, so
+// the device compiler cannot fuse one of them into the add (-ffp-contract=fast
+// ignores contraction pragmas)
+MFD double lincomb_rn(double a, double x, double b, double y) {
+    double p = a * x, q = b * y;
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(p), "+v"(q));
+#endif
+    return p + q;
+}
+
 template <class P>
 MFD void dt_partial(const Tab &T, const double *S, int do_ns, double &ws, double &maxd) {
     constexpr int NS = P::NS, NV = Num<P>::NV;
@@ -924,6 +935,8 @@ __global__ __launch_bounds__(256) void k_post(ResArgs r, Work w) {
                 for (int s = 0; s < NS; ++s) if (S[FLO + s] < 0.0) S[FLO + s] = 0.0;
             }
             if (r.track_res) {
+                // LinComb(a, S_new, b, S_old) rounded as written (no fused
+                // multiply-add), so that an unchanged component gives exactly 0
                 const double ia = 1.0 / dt, ib = -1.0 / dt;
                 if (CHEM) {
                     // conserved components and vfrac: S_new == S_old, exactly zero
@@ -931,7 +944,7 @@ __global__ __launch_bounds__(256) void k_post(ResArgs r, Work w) {
 #pragma unroll
                     for (int n = DENS; n < NV; ++n) {
                         if (n == VFRAC) continue;
-                        double df = ia * S[n] + ib * So_p[n - DENS];
+                        double df = lincomb_rn(ia, S[n], ib, So_p[n - DENS]);
                         if (n < VFRAC) df *= vfo;
                         sums[n] += df * df;
                     }
@@ -941,7 +954,7 @@ __global__ __launch_bounds__(256) void k_post(ResArgs r, Work w) {
                     const double vfo = op[(int64_t)VFRAC * o.ns];
 #pragma unroll
                     for (int n = 0; n < NV; ++n) {
-                        double df = ia * S[n] + ib * op[(int64_t)n * o.ns];
+                        double df = lincomb_rn(ia, S[n], ib, op[(int64_t)n * o.ns]);
                         if (n < VFRAC) df *= vfo;
                         sums[n] += df * df;
                     }
@@ -1403,6 +1416,88 @@ __global__ __launch_bounds__(128) void k_cutcells_box(CutArgs a) {
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Chemistry phase of the split mode (mflo_splitchem.cpp:40-62), one thread per
+// valid cell: nsub substeps of Advance_chemistry (:172-195), an explicit
+// midpoint rule whose first stage the reference evaluates on phi_new after
+// its swap -- the state before the previous step or substep, held here in the
+// lag buffer -- and its second on the half-step state; then, with finish,
+// update_vars_after_chemsolve (:74-140; no background inert gas). Only the
+// species have a right-hand side, so LinComb leaves every other component
+// equal to the current state; temperature is constant over the phase.
+// Consecutive launches continue the substep sequence exactly (the cell's
+// state is its current and its lag buffer).
+// ---------------------------------------------------------------------------
+template <class P>
+__global__ __launch_bounds__(256) void k_chem_split(ChemSplitArgs c) {
+    using N = Num<P>;
+    constexpr int NS = P::NS, NV = N::NV;
+    const Tab &T = *c.tab;
+    const int nx = c.n[0], ny = c.n[1];
+    const int64_t ncell = (int64_t)nx * ny * c.n[2];
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < ncell; q += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((int64_t)nx * ny));
+        double *sp = c.s.p + c.s.off(i, j, k);
+        double *lp = c.lag.p + c.lag.off(i, j, k);
+        const int64_t sn = c.s.ns, ln = c.lag.ns;
+        double S[NV];
+#pragma unroll
+        for (int n = 0; n < NV; ++n) S[n] = sp[(int64_t)n * sn];
+        if (P::kHasChemistry && c.nsub > 0) {
+            double L[NV];
+#pragma unroll
+            for (int n = 0; n < NV; ++n) L[n] = lp[(int64_t)n * ln];
+            double u[NS], prev[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) u[s] = S[FLO + s];
+            const double hdt = kHalf * c.dt;
+            for (int m = 0; m < c.nsub; ++m) {
+                double src[NS], st[NS];
+                // stage 1 on the lag state (all its components)
+                P::spec_source(T, ChemCell<NS>{L, L + FLO}, src);
+#pragma unroll
+                for (int s = 0; s < NS; ++s) st[s] = kOne * u[s] + hdt * (kZero + src[s]);
+                // stage 2 on the half-step state (current non-species components)
+                P::spec_source(T, ChemCell<NS>{S, st}, src);
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    prev[s] = u[s];
+                    u[s] = kOne * u[s] + c.dt * (kZero + src[s]);
+                }
+                // the state before this substep becomes the lag state
+#pragma unroll
+                for (int n = 0; n < NV; ++n) L[n] = S[n];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) L[FLO + s] = prev[s];
+            }
+#pragma unroll
+            for (int s = 0; s < NS; ++s) S[FLO + s] = u[s];
+#pragma unroll
+            for (int n = 0; n < NV; ++n) lp[(int64_t)n * ln] = L[n];
+        }
+        if (c.finish && S[VFRAC] >= kOne) {
+            const double rho = P::get_r_from_c(T, S + FLO);
+            S[RHO] = rho;
+            S[DENS] = rho;
+            S[RHOU] = rho * S[VELX];
+            S[RHOV] = rho * S[VELY];
+            S[RHOW] = rho * S[VELZ];
+            double uu[N::NC], p[NCV];
+#pragma unroll
+            for (int n = 0; n < NCV; ++n) uu[n] = S[n];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) uu[NCV + s] = S[FLO + s];
+            N::cons_to_prim(T, uu, p);
+#pragma unroll
+            for (int n = 0; n < NCV; ++n) S[DENS + n] = p[n];
+            S[TEMP] = P::get_t_from_rpc(T, S[DENS], S[PRES], uu + NCV);
+        }
+        if (c.finish && c.do_ns && S[PRES] < 0) raise_err(c.err, MF_ERR_NEG_PRESSURE);
+#pragma unroll
+        for (int n = 0; n < NV; ++n) sp[(int64_t)n * sn] = S[n];
+    }
+}
+
 template <class P>
 struct Launch {
     static void fill(const FillArgs &a, int phase, hipStream_t st) {
@@ -1471,6 +1566,11 @@ struct Launch {
     static void est(const ResArgs &r, const Work &w, int grid, int block, hipStream_t st) {
         hipLaunchKernelGGL(k_est<P>, dim3(grid), dim3(block), 0, st, r, w);
     }
+    static void chem_split(const ChemSplitArgs &a, hipStream_t st) {
+        const int64_t ncell = (int64_t)a.n[0] * a.n[1] * a.n[2];
+        const int64_t g = std::min<int64_t>((ncell + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_chem_split<P>, dim3((unsigned)std::max<int64_t>(g, 1)), dim3(256), 0, st, a);
+    }
     static KernelSet set(const AdjSet *adj = nullptr) {
         KernelSet k;
         k.ns = P::NS;
@@ -1485,6 +1585,7 @@ struct Launch {
         k.residual = &residual;
         k.est = &est;
         k.post = &post;
+        k.chem_split = &chem_split;
         return k;
     }
 };

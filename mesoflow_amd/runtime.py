@@ -41,6 +41,10 @@ SIGNATURES = {
     "mf_est_dt": ([C.c_void_p, _P(C.c_double)], C.c_int),
     "mf_level_step": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "mf_level_step_timed": ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
+    "mf_level_evolve": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, _P(C.c_int)], C.c_int),
+    "mf_level_advance_chemistry": ([C.c_void_p, C.c_double, C.c_double, C.c_double], C.c_int),
+    "mf_level_evolve_split": ([C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int],
+                              C.c_int),
     "mf_level_info": ([C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_int), C.c_void_p], C.c_int),
     "mf_level_set_time": ([C.c_void_p, C.c_double, C.c_double, C.c_int], C.c_int),
     "mf_level_set_tile": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
@@ -166,6 +170,23 @@ class Level:
         check(lib().mf_level_step(self.h, int(n), int(mode)), "mf_level_step")
 
     KERNEL_CLASSES = ("compute_dt", "fill_ghosts", "cutcells", "face_flux", "residual_combine", "finalize", "post_step")
+
+    def evolve(self, final_time, max_step=2 ** 30, only_flow=True, mode=abi.MODE_BATCHED):
+        """EvolveAMR(final_time, only_flow); returns the number of steps taken."""
+        n = C.c_int()
+        check(lib().mf_level_evolve(self.h, float(final_time), int(max_step), int(bool(only_flow)), int(mode), C.byref(n)),
+              "mf_level_evolve")
+        return n.value
+
+    def advance_chemistry(self, react_time, dt_react, dt_react_rk):
+        """Chemistry phase of the split mode (substeps + update_vars_after_chemsolve)."""
+        check(lib().mf_level_advance_chemistry(self.h, float(react_time), float(dt_react), float(dt_react_rk)),
+              "mf_level_advance_chemistry")
+
+    def evolve_split(self, t_ss, t_react, dt_react, dt_react_rk, dt_ss, max_step=2 ** 30, mode=abi.MODE_BATCHED):
+        """Evolve_split: flow to t_ss, then chemistry phases and flow-coupling intervals."""
+        check(lib().mf_level_evolve_split(self.h, float(t_ss), float(t_react), float(dt_react), float(dt_react_rk),
+                                          float(dt_ss), int(max_step), int(mode)), "mf_level_evolve_split")
 
     def step_timed(self, n=1, mode=abi.MODE_BATCHED):
         """Steps with HIP events around each kernel class; returns {class: (ms, launches)}."""

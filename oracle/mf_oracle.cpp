@@ -86,6 +86,11 @@ struct Box { int lo[3], hi[3]; };
 struct ShockTubeN2 {
     static const int NS = 1;
     const mf_plugin_params *T;
+    double get_r_from_c(const double *spec) const {  // thermo.H:39-48
+        double rho = zeroval;
+        for (int sp = 0; sp < NS; sp++) rho += spec[sp] * T->molwt[sp];
+        return rho;
+    }
     double get_t_from_rpc(double rho, double pres, const double *spec) const {
         (void)rho;
         double sum_ci = zeroval;
@@ -930,6 +935,9 @@ struct LevelBase {
     virtual void get_state(double *s) const = 0;
     virtual int est_dt(double *dt) = 0;
     virtual int fill_ghosts_to(double *padded_out, double time) = 0;
+    virtual int evolve(double final_time, int max_step, int only_flow, int *nsteps_out) = 0;
+    virtual int advance_chemistry_phase(double react_time, double dt_react, double dt_rk) = 0;
+    virtual int evolve_split(double t_ss, double t_react, double dt_react, double dt_rk, double dt_ss, int max_step) = 0;
     virtual int nv() const = 0;
     mf_params prm;
     mf_geom geom;
@@ -1169,9 +1177,10 @@ struct Level : LevelBase {
             v = phi_old[q] + two * v;     // Xpay: dst = src + a*dst
             phi_new[q] = v * third;       // mult
         }
-        if (prm.do_chemistry) advance_chemistry(time, dt_lev);
+        if (prm.do_chemistry && !only_flow) advance_chemistry(time, dt_lev);
         return MF_OK;
     }
+    int only_flow = 0;  // EvolveAMR(final_time, only_flow) of the split mode
     // Advance_chemistry_implicit with AMReX's explicit RK, rk.type = 3 (SSPRK3):
     // nodes {0,1,1/2}, tableau {{},{1},{1/4,1/4}}, weights {1/6,1/6,2/3};
     // stage state = S_old + sum_j (dt*a_ij) F_j; S_new = S_old + sum_i (dt*w_i) F_i.
@@ -1254,6 +1263,110 @@ struct Level : LevelBase {
             }
             resnorms[c] = std::sqrt(sum);
         }
+    }
+    // Source/mflo_splitchem.cpp:172-195 (Advance_chemistry): explicit midpoint
+    // rule. After the swap the reference evaluates the first stage on S_new =
+    // phi_new, the array that held the state before the previous step or
+    // substep, not the current state (S_old); kept.
+    void advance_chemistry_split(double time, double dt_lev) {
+        (void)time;
+        std::swap(phi_old, phi_new);
+        const long nc = ncell();
+#pragma omp parallel for schedule(static)
+        for (long q = 0; q < nc; ++q) {
+            double S[FLO_NVARS + 16], src[FLO_NVARS + 16], d[FLO_NVARS + 16];
+            // stage 1: compute_dsdt_chemistry(S_new) (mflo_evolve.cpp:684-720), LinComb
+            for (int c = 0; c < NV; ++c) S[c] = phi_new[(size_t)c * nc + q];
+            for (int c = 0; c < NV; ++c) src[c] = zeroval;
+            pl.spec_source(S, src);
+            for (int c = 0; c < NV; ++c) d[c] = zeroval + src[c];
+            for (int c = 0; c < NV; ++c) phi_new[(size_t)c * nc + q] = one * phi_old[(size_t)c * nc + q] + (half * dt_lev) * d[c];
+            // stage 2 on the half-step state
+            for (int c = 0; c < NV; ++c) S[c] = phi_new[(size_t)c * nc + q];
+            for (int c = 0; c < NV; ++c) src[c] = zeroval;
+            pl.spec_source(S, src);
+            for (int c = 0; c < NV; ++c) d[c] = zeroval + src[c];
+            for (int c = 0; c < NV; ++c) phi_new[(size_t)c * nc + q] = one * phi_old[(size_t)c * nc + q] + dt_lev * d[c];
+        }
+    }
+    // Source/mflo_splitchem.cpp:74-140 (update_vars_after_chemsolve), no
+    // background inert gas
+    int update_vars_after_chemsolve() {
+        Ops<P> op(pl);
+        const long nc = ncell();
+        double minp_all = std::numeric_limits<double>::max();
+        double *s = phi_new.data();
+#pragma omp parallel for schedule(static)
+        for (long q = 0; q < nc; ++q) {
+            if (s[(size_t)VFRAC * nc + q] >= one) {
+                double u[NCVARS + 16], p[NCVARS], spec[16];
+                for (int c = 0; c < NS; c++) spec[c] = s[(size_t)(c + FLO_NVARS) * nc + q];
+                s[(size_t)RHO * nc + q] = pl.get_r_from_c(spec);
+                s[(size_t)DENS * nc + q] = s[(size_t)RHO * nc + q];
+                s[(size_t)RHOU * nc + q] = s[(size_t)RHO * nc + q] * s[(size_t)VELX * nc + q];
+                s[(size_t)RHOV * nc + q] = s[(size_t)RHO * nc + q] * s[(size_t)VELY * nc + q];
+                s[(size_t)RHOW * nc + q] = s[(size_t)RHO * nc + q] * s[(size_t)VELZ * nc + q];
+                for (int c = 0; c < NCVARS; c++) u[c] = s[(size_t)c * nc + q];
+                for (int c = 0; c < NS; c++) u[c + NCVARS] = s[(size_t)(c + FLO_NVARS) * nc + q];
+                op.cons_to_prim(u, p);
+                for (int c = 0; c < NCVARS; c++) s[(size_t)(c + DENS) * nc + q] = p[c];
+                s[(size_t)TEMP * nc + q] = pl.get_t_from_rpc(s[(size_t)DENS * nc + q], s[(size_t)PRES * nc + q], u + NCVARS);
+            }
+        }
+#pragma omp parallel for schedule(static) reduction(min : minp_all)
+        for (long q = 0; q < nc; ++q) minp_all = std::min(minp_all, s[(size_t)PRES * nc + q]);
+        return (minp_all < 0 && prm.do_ns) ? MF_ERR_NEG_PRESSURE : MF_OK;
+    }
+    // the chemistry phase of Evolve_split (Source/mflo_splitchem.cpp:38-62,
+    // single level, no background inert gas)
+    int advance_chemistry_phase(double react_time, double dt_react, double dt_rk) override {
+        if (prm.using_bg_inertgas) return MF_ERR_UNSUPPORTED;
+        double tchem = 0.0;
+        while (tchem < dt_react) {
+            advance_chemistry_split(react_time + tchem, dt_rk);
+            tchem += dt_rk;
+        }
+        return update_vars_after_chemsolve();
+    }
+    // Source/mflo_evolve.cpp:18-89: EvolveAMR(final_time, only_flow), no I/O
+    int evolve(double final_time, int max_step, int only_flow_, int *nsteps_out) override {
+        double cur_time = t_new;
+        int taken = 0, st = MF_OK;
+        only_flow = only_flow_;
+        for (int s = istep; s < max_step && cur_time < final_time; ++s) {
+            compute_dt();
+            st = advance(cur_time, dt);
+            if (st) break;
+            ++istep;
+            ++taken;
+            st = update_primitive_vars();
+            if (st) break;
+            if (prm.clip_species) clip_neg_speciesconc();
+            if (prm.track_residual_norms) compute_residual_norms();
+            cur_time += dt;
+            t_new = cur_time;
+            if (cur_time >= prm.stop_time - 1.e-6 * dt) break;
+        }
+        only_flow = 0;
+        if (nsteps_out) *nsteps_out = taken;
+        return st;
+    }
+    // Source/mflo_splitchem.cpp:24-72 (Evolve_split)
+    int evolve_split(double t_ss, double t_react, double dt_react, double dt_rk, double dt_ss, int max_step) override {
+        if (prm.using_bg_inertgas) return MF_ERR_UNSUPPORTED;
+        double react_time = 0.0, flow_time;
+        int st = evolve(t_ss, max_step, 1, nullptr);
+        if (st) return st;
+        flow_time = t_ss;
+        while (react_time < t_react) {
+            st = advance_chemistry_phase(react_time, dt_react, dt_rk);
+            if (st) return st;
+            st = evolve(flow_time + dt_ss, max_step, 1, nullptr);
+            if (st) return st;
+            react_time += dt_react;
+            flow_time += dt_ss;
+        }
+        return MF_OK;
     }
     // Source/mflo_evolve.cpp:18-89 (nsteps coarse steps, no I/O)
     int step(int nsteps) override {
@@ -1348,6 +1461,15 @@ void orc_level_destroy(void *L) { delete (LevelBase *)L; }
 void orc_level_set_state(void *L, const double *s) { ((LevelBase *)L)->set_state(s); }
 void orc_level_get_state(void *L, double *s) { ((LevelBase *)L)->get_state(s); }
 int orc_level_step(void *L, int nsteps) { return ((LevelBase *)L)->step(nsteps); }
+int orc_level_evolve(void *L, double final_time, int max_step, int only_flow, int *nsteps_out) {
+    return ((LevelBase *)L)->evolve(final_time, max_step, only_flow, nsteps_out);
+}
+int orc_level_advance_chemistry(void *L, double react_time, double dt_react, double dt_rk) {
+    return ((LevelBase *)L)->advance_chemistry_phase(react_time, dt_react, dt_rk);
+}
+int orc_level_evolve_split(void *L, double t_ss, double t_react, double dt_react, double dt_rk, double dt_ss, int max_step) {
+    return ((LevelBase *)L)->evolve_split(t_ss, t_react, dt_react, dt_rk, dt_ss, max_step);
+}
 int orc_level_est_dt(void *L, double *dt) { return ((LevelBase *)L)->est_dt(dt); }
 int orc_level_fill_ghosts(void *L, double *padded, double time) { return ((LevelBase *)L)->fill_ghosts_to(padded, time); }
 void orc_level_info(void *L, double *t, double *dt, int *istep, double *res) {

@@ -39,6 +39,13 @@ def lib():
         L.orc_level_get_state.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_level_step.argtypes = [C.c_void_p, C.c_int]
         L.orc_level_step.restype = C.c_int
+        L.orc_level_evolve.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, P(C.c_int)]
+        L.orc_level_evolve.restype = C.c_int
+        L.orc_level_advance_chemistry.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double]
+        L.orc_level_advance_chemistry.restype = C.c_int
+        L.orc_level_evolve_split.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                             C.c_int]
+        L.orc_level_evolve_split.restype = C.c_int
         L.orc_level_est_dt.argtypes = [C.c_void_p, P(C.c_double)]
         L.orc_level_est_dt.restype = C.c_int
         L.orc_level_fill_ghosts.argtypes = [C.c_void_p, C.c_void_p, C.c_double]
@@ -94,6 +101,25 @@ class OracleLevel:
         st = lib().orc_level_step(self.h, int(n))
         if st != 0:
             raise abi.MFError("orc_level_step", st)
+
+    def evolve(self, final_time, max_step=2 ** 30, only_flow=True):
+        """EvolveAMR(final_time, only_flow); returns the number of steps taken."""
+        n = C.c_int()
+        st = lib().orc_level_evolve(self.h, float(final_time), int(max_step), int(bool(only_flow)), C.byref(n))
+        if st != 0:
+            raise abi.MFError("orc_level_evolve", st)
+        return n.value
+
+    def advance_chemistry(self, react_time, dt_react, dt_react_rk):
+        st = lib().orc_level_advance_chemistry(self.h, float(react_time), float(dt_react), float(dt_react_rk))
+        if st != 0:
+            raise abi.MFError("orc_level_advance_chemistry", st)
+
+    def evolve_split(self, t_ss, t_react, dt_react, dt_react_rk, dt_ss, max_step=2 ** 30):
+        st = lib().orc_level_evolve_split(self.h, float(t_ss), float(t_react), float(dt_react), float(dt_react_rk),
+                                          float(dt_ss), int(max_step))
+        if st != 0:
+            raise abi.MFError("orc_level_evolve_split", st)
 
     def est_dt(self):
         d = C.c_double()
