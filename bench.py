@@ -158,6 +158,30 @@ def adjoint_bench(nsteps=200, ncheck_binomial=20):
     return out
 
 
+def substeps(dt_react, dt_rk):
+    """Substeps of the reference's `while (tchem < dt_react) {...; tchem += dt_rk;}`."""
+    n, t = 0, 0.0
+    while t < dt_react:
+        n += 1
+        t += dt_rk
+    return n
+
+
+def split_chem_cpu(name, nsub=100, dt_rk=1.0e-6):
+    """Chemistry phase of the split mode on the oracle (64^3 sample of the workload)."""
+    from mesoflow_amd import cases
+    from oracle.pyoracle import OracleLevel
+    c = cases.porous_reactor(n_cell=(64, 64, 64), nspec=5 if name != "porous512" else 10,
+                             nreact=2 if name != "porous512" else 8, geometry="spheres", porosity=0.6, rmin=6.0, rmax=14.0,
+                             order_hyp=5, cfl=0.3, seed=20261203)
+    L = OracleLevel(c)
+    L.set_state(c.make_state())
+    t0 = time.time()
+    L.advance_chemistry(0.0, nsub * dt_rk, dt_rk)
+    secs = time.time() - t0
+    return 64 ** 3 * substeps(nsub * dt_rk, dt_rk) / secs
+
+
 def read_traffic(config):
     path = os.path.join(ROOT, "profiles", "traffic_%s.txt" % config)
     if not os.path.exists(path):
@@ -182,6 +206,8 @@ def main():
     ap.add_argument("--calibrate", action="store_true", help="plain copy kernel for PMC scale factors")
     ap.add_argument("--compare-steps", type=int, default=3,
                     help="steps timed in each launch mode after the main region (0 = skip)")
+    ap.add_argument("--split-substeps", type=int, default=2000,
+                    help="chemistry substeps of the split-mode measurement (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the discrete-adjoint measurement at 128^3 (0 = skip)")
     args = ap.parse_args()
@@ -281,6 +307,28 @@ def main():
         extra["ms_per_step_by_mode"] = modes
         extra["batched_speedup_vs_per_box"] = modes["per_box"] / modes["batched"]
         extra["graph_speedup_vs_per_box"] = modes["per_box"] / modes["graph"]
+
+    if args.split_substeps > 0:
+        # split-chemistry mode (Evolve_split): one flow-only step gives the
+        # chemistry phase its lag state, then one phase of split_substeps
+        # explicit-midpoint substeps of every cell, fused in one kernel per
+        # 1024 substeps
+        t_now, dt_now = lvl.info()[:2]
+        lvl.evolve(t_now + 0.5 * dt_now, only_flow=True)
+        torch.cuda.synchronize()
+        dt_rk = 1.0e-6
+        nsub = substeps(args.split_substeps * dt_rk, dt_rk)
+        ta = time.perf_counter()
+        lvl.advance_chemistry(0.0, args.split_substeps * dt_rk, dt_rk)
+        torch.cuda.synchronize()
+        secs_c = time.perf_counter() - ta
+        sc = {"substeps": nsub, "dt_rk": dt_rk, "ms": secs_c * 1e3, "cell_substeps_per_s": cells * nsub / secs_c}
+        if not args.no_cpu_baseline:
+            sc["cpu_cell_substeps_per_s"] = split_chem_cpu(args.config)
+            sc["cpu_sample"] = "oracle chemistry phase on the 64^3 sample, 100 substeps, OMP threads %s" % os.environ.get(
+                "OMP_NUM_THREADS", os.cpu_count())
+        extra["split_chemistry"] = sc
+        log("split chemistry: %s" % json.dumps(sc))
 
     if args.adjoint_steps > 0:
         lvl.close()

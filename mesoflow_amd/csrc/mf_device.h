@@ -139,6 +139,10 @@ struct PShockTubeN2 {
     static constexpr bool kHasChemistry = false;
     static constexpr bool kGammaIgnoresRhoP = true;  // gamma depends on the species only
     template <class C> MFD static void spec_source(const Tab &, const C &, double *src) { src[0] = kZero; }
+    MFD static void rate_constants(const Tab &, double, double *) {}
+    template <class C> MFD static void spec_source_k(const Tab &T, const double *, const C &cell, double *src) {
+        spec_source(T, cell, src);
+    }
     MFD static void externalbc(const Tab &T, const double *x, const double *s_int, double *s_ext, int idir, int sgn,
                                double time) {
         s_ext[PRES] = s_int[PRES];
@@ -224,6 +228,11 @@ struct PCatalystCylinder : PMixture<4, 3> {
         src[1] -= reactrate;
         src[2] += reactrate;
     }
+    // a constant rate: nothing depends on temperature
+    MFD static void rate_constants(const Tab &, double, double *) {}
+    template <class C> MFD static void spec_source_k(const Tab &T, const double *, const C &cell, double *src) {
+        spec_source(T, cell, src);
+    }
     MFD static void externalbc(const Tab &T, const double *x, const double *s_int, double *s_ext, int idir, int sgn,
                                double time) {
         double spec[NS] = {kZero, kZero, kZero, kZero};
@@ -267,29 +276,51 @@ struct PPorous : PMixture<NSv, NSv - 1> {
     static constexpr int NS = NSv;
     static constexpr int NG = NSv - 1;
     static constexpr bool kHasChemistry = true;
+    // mass-action rate of reaction r with rate constant krate, added to wdot
+    template <class C> MFD static void add_reaction(const Tab &T, int r, double krate, const C &cell, double *wdot) {
+        double concmult = 1.0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double nu = T.stoich[r][s];
+            if (nu < 0.0) {
+                double c = cell(FLO + s);
+                double a = fabs(nu);
+                concmult *= (a == 1.0) ? c : ((a == 2.0) ? c * c : pow(c, a));
+            }
+        }
+        double reactrate = krate * concmult;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            double nu = T.stoich[r][s];
+            if (fabs(nu) > 0.0) wdot[s] += nu * reactrate;
+        }
+    }
     template <class C> MFD static void spec_source(const Tab &T, const C &cell, double *src) {
         double wdot[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) wdot[s] = kZero;
         double Tm = cell(TEMP);
-        for (int r = 0; r < T.nreact; ++r) {
-            double concmult = 1.0;
+        for (int r = 0; r < T.nreact; ++r) add_reaction(T, r, exp(T.log_A[r] - T.Ea[r] / (T.R_univ * Tm)), cell, wdot);
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                double nu = T.stoich[r][s];
-                if (nu < 0.0) {
-                    double c = cell(FLO + s);
-                    double a = fabs(nu);
-                    concmult *= (a == 1.0) ? c : ((a == 2.0) ? c * c : pow(c, a));
-                }
-            }
-            double krate = exp(T.log_A[r] - T.Ea[r] / (T.R_univ * Tm));
-            double reactrate = krate * concmult;
+        for (int s = 0; s < NS; ++s) src[s] = kZero + wdot[s];
+    }
+    // the temperature-only part of the source, k_r = A_r exp(-Ea_r / (R T)),
+    // and the source from given rate constants (the same values as spec_source)
+    MFD static void rate_constants(const Tab &T, double Tm, double *k) {
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                double nu = T.stoich[r][s];
-                if (fabs(nu) > 0.0) wdot[s] += nu * reactrate;
-            }
+        for (int r = 0; r < MF_MAX_REACTIONS; ++r) {
+            k[r] = kZero;
+            if (r < T.nreact) k[r] = exp(T.log_A[r] - T.Ea[r] / (T.R_univ * Tm));
+        }
+    }
+    template <class C> MFD static void spec_source_k(const Tab &T, const double *k, const C &cell, double *src) {
+        double wdot[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) wdot[s] = kZero;
+#pragma unroll
+        for (int r = 0; r < MF_MAX_REACTIONS; ++r) {
+            if (r >= T.nreact) break;
+            add_reaction(T, r, k[r], cell, wdot);
         }
 #pragma unroll
         for (int s = 0; s < NS; ++s) src[s] = kZero + wdot[s];

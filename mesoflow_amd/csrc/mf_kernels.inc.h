@@ -1451,14 +1451,21 @@ __global__ __launch_bounds__(256) void k_chem_split(ChemSplitArgs c) {
 #pragma unroll
             for (int s = 0; s < NS; ++s) u[s] = S[FLO + s];
             const double hdt = kHalf * c.dt;
+            // rate constants depend on temperature only: those of the current
+            // state, and of the lag state for the first substep (later lag
+            // states carry the current temperature); the same values the
+            // source would recompute at every stage
+            double kS[MF_MAX_REACTIONS], kL[MF_MAX_REACTIONS];
+            P::rate_constants(T, S[TEMP], kS);
+            P::rate_constants(T, L[TEMP], kL);
             for (int m = 0; m < c.nsub; ++m) {
                 double src[NS], st[NS];
                 // stage 1 on the lag state (all its components)
-                P::spec_source(T, ChemCell<NS>{L, L + FLO}, src);
+                P::spec_source_k(T, m == 0 ? kL : kS, ChemCell<NS>{L, L + FLO}, src);
 #pragma unroll
                 for (int s = 0; s < NS; ++s) st[s] = kOne * u[s] + hdt * (kZero + src[s]);
                 // stage 2 on the half-step state (current non-species components)
-                P::spec_source(T, ChemCell<NS>{S, st}, src);
+                P::spec_source_k(T, kS, ChemCell<NS>{S, st}, src);
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
                     prev[s] = u[s];
