@@ -496,6 +496,11 @@ int grid_for(int ntiles, int max_grid) {
 
 bool use_lds(Level *L) { return L->c->prm.order_hyp == 5 && L->c->prm.do_ns; }
 
+// threads of an LDS face-sweep block: 512 for tiles of at least 512 faces (a
+// box-end tile has one more face per row along the sweep, which at 256 faces
+// per block leaves a mostly idle extra wave pass), else 256
+int face_block(const int *t) { return t[0] * t[1] * t[2] >= 512 ? 512 : 256; }
+
 void launch_faces(Level *L, const StageArgs &a, int dir, int mode) {
     Ctx *c = L->c;
     const bool lds = use_lds(L);
@@ -503,7 +508,7 @@ void launch_faces(Level *L, const StageArgs &a, int dir, int mode) {
     Work w = level_work(L, cls);
     const size_t shm = lds ? c->ks->lds_bytes(L->tsh[cls], dir) : 0;
     auto go = [&](int grid) {
-        if (lds) c->ks->faces_lds(a, w, dir, grid, 256, shm, c->stream);
+        if (lds) c->ks->faces_lds(a, w, dir, grid, face_block(L->tsh[cls]), shm, c->stream);
         else c->ks->faces(a, w, dir, grid, 256, c->stream);
     };
     if (mode == MF_MODE_PER_BOX) {
@@ -1723,9 +1728,9 @@ int mf_level_autotune(void *level) {
     if (!L) return MF_ERR_BAD_ARG;
     Ctx *c = L->c;
     static const int face_cand[3][8][3] = {
-        {{64, 4, 1}, {64, 2, 2}, {128, 2, 1}, {32, 4, 2}, {32, 8, 1}, {64, 1, 4}, {128, 1, 2}, {32, 2, 4}},
-        {{32, 8, 1}, {16, 16, 1}, {64, 4, 1}, {32, 4, 2}, {16, 8, 2}, {64, 8, 1}, {32, 16, 1}, {16, 12, 2}},
-        {{32, 1, 8}, {16, 2, 8}, {32, 2, 4}, {64, 1, 4}, {16, 1, 16}, {64, 1, 8}, {32, 1, 16}, {16, 2, 12}}};
+        {{64, 4, 1}, {64, 2, 2}, {32, 16, 1}, {32, 4, 2}, {32, 8, 1}, {64, 1, 4}, {32, 8, 2}, {32, 2, 4}},
+        {{32, 8, 1}, {16, 16, 1}, {16, 16, 2}, {32, 4, 2}, {16, 8, 2}, {64, 8, 1}, {32, 16, 1}, {16, 12, 2}},
+        {{32, 1, 8}, {16, 2, 8}, {32, 2, 4}, {16, 2, 16}, {16, 1, 16}, {64, 1, 8}, {32, 1, 16}, {16, 2, 12}}};
     static const int cell_cand[8][3] = {{64, 2, 2}, {32, 4, 2}, {64, 4, 1}, {128, 2, 1}, {32, 8, 1}, {16, 4, 4}, {32, 2, 4}, {64, 1, 4}};
     HIPCHK(hipStreamSynchronize(c->stream));
     TimeState saved;

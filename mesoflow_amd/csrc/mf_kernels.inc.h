@@ -419,12 +419,13 @@ struct LdsCell {
     }
 };
 
-// minimum waves per SIMD the compiler must keep room for (register budget)
+// minimum waves per SIMD the compiler must keep room for (register budget);
+// blocks of 256 or 512 threads (512 for tiles of at least 512 faces)
 #ifndef MF_FACE_MIN_WAVES
 #define MF_FACE_MIN_WAVES 2
 #endif
 template <class P, int D>
-__global__ __launch_bounds__(256, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs a, Work w) {
+__global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs a, Work w) {
     using N = Num<P>;
     constexpr int NS = P::NS, NC = N::NC;
     constexpr int e0 = D == 0, e1 = D == 1, e2 = D == 2;
@@ -829,6 +830,9 @@ __global__ __launch_bounds__(256) void k_residual(ResArgs r, Work w) {
                 pd = r.dsdt.p + r.dsdt.off(i, j, k);
                 dn = r.dsdt.ns;
             }
+            // all loads first, then the stores: a store inside the component loop
+            // would keep the next component's loads behind it (the arrays may alias)
+            double Sout[NV];
 #pragma unroll
             for (int n = 0; n < NV; ++n) {
                 double d;
@@ -863,8 +867,10 @@ __global__ __launch_bounds__(256) void k_residual(ResArgs r, Work w) {
                     v = o + kTwo * v;
                     S = v * kThird;
                 }
-                pd[(int64_t)n * dn] = S;
+                Sout[n] = S;
             }
+#pragma unroll
+            for (int n = 0; n < NV; ++n) pd[(int64_t)n * dn] = Sout[n];
         }
     }
 }
