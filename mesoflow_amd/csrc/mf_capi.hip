@@ -1727,10 +1727,10 @@ int mf_level_autotune(void *level) {
     Level *L = (Level *)level;
     if (!L) return MF_ERR_BAD_ARG;
     Ctx *c = L->c;
-    static const int face_cand[3][8][3] = {
-        {{64, 4, 1}, {64, 2, 2}, {32, 16, 1}, {32, 4, 2}, {32, 8, 1}, {64, 1, 4}, {32, 8, 2}, {32, 2, 4}},
-        {{32, 8, 1}, {16, 16, 1}, {16, 16, 2}, {32, 4, 2}, {16, 8, 2}, {64, 8, 1}, {32, 16, 1}, {16, 12, 2}},
-        {{32, 1, 8}, {16, 2, 8}, {32, 2, 4}, {16, 2, 16}, {16, 1, 16}, {64, 1, 8}, {32, 1, 16}, {16, 2, 12}}};
+    static const int face_cand[3][10][3] = {
+        {{64, 4, 1}, {64, 2, 2}, {32, 16, 1}, {32, 4, 2}, {32, 8, 1}, {64, 1, 4}, {32, 8, 2}, {32, 2, 4}, {32, 4, 4}, {32, 2, 8}},
+        {{32, 8, 1}, {16, 16, 1}, {16, 16, 2}, {32, 4, 2}, {16, 8, 2}, {64, 8, 1}, {32, 16, 1}, {16, 12, 2}, {8, 32, 2}, {16, 32, 1}},
+        {{32, 1, 8}, {16, 2, 8}, {32, 2, 4}, {16, 2, 16}, {16, 1, 16}, {64, 1, 8}, {32, 1, 16}, {16, 2, 12}, {8, 2, 32}, {16, 1, 32}}};
     static const int cell_cand[8][3] = {{64, 2, 2}, {32, 4, 2}, {64, 4, 1}, {128, 2, 1}, {32, 8, 1}, {16, 4, 4}, {32, 2, 4}, {64, 1, 4}};
     HIPCHK(hipStreamSynchronize(c->stream));
     TimeState saved;
@@ -1772,7 +1772,7 @@ int mf_level_autotune(void *level) {
     const bool lds = use_lds(L);
     for (int cls = 0; cls <= kCellClass && st == MF_OK; ++cls) {
         if (cls < 3 && !lds) continue;
-        const int ncand = 8;
+        const int ncand = cls < 3 ? 10 : 8;
         double best = 1e300;
         int bi[3] = {L->tsh[cls][0], L->tsh[cls][1], L->tsh[cls][2]};
         for (int q = 0; q < ncand; ++q) {

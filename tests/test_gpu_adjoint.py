@@ -30,8 +30,8 @@ def say(*a):
     print(*a, flush=True)
 
 
-def adj_case(n_cell, geometry="spheres", nreact=3, seed=5, boxes=None, strong=True):
-    c = cases.porous_reactor(n_cell=n_cell, nspec=5, nreact=nreact, geometry=geometry, porosity=0.6, rmin=2.0, rmax=3.5,
+def adj_case(n_cell, geometry="spheres", nreact=3, seed=5, boxes=None, strong=True, nspec=5):
+    c = cases.porous_reactor(n_cell=n_cell, nspec=nspec, nreact=nreact, geometry=geometry, porosity=0.6, rmin=2.0, rmax=3.5,
                              order_hyp=5, cfl=0.3, seed=seed, boxes=boxes)
     if strong:
         # fast chemistry so that the rate constants visibly change J within a few steps
@@ -50,7 +50,8 @@ def adj_case(n_cell, geometry="spheres", nreact=3, seed=5, boxes=None, strong=Tr
 def perturbed_state(case, seed=1, amp=0.02):
     s = case.make_state()
     rng = np.random.default_rng(seed)
-    for q in (0, 1, 2, 3, 4, 12, 13, 14, 15):
+    # conserved components and the gas species (the last species is the site)
+    for q in list(range(5)) + list(range(12, 12 + case.nspec - 1)):
         s[q] *= 1.0 + amp * rng.uniform(-1.0, 1.0, s[q].shape)
     s[1:4] += amp * rng.uniform(-1.0, 1.0, s[1:4].shape) * s[0]
     s[6:9] = s[1:4] / s[0]
@@ -104,6 +105,40 @@ def test_adjoint_initial_state_directional_derivative(variant):
     ad = float(np.sum(lam * d))
     say("J %.12e  <dJ/dU0, d> %.12e  fd %.12e" % (J0, ad, fd))
     assert abs(ad - fd) <= 1e-6 * abs(fd) + 1e-18 * abs(J0), (ad, fd)
+    L.close()
+    ctx.close()
+
+
+def test_adjoint_ns10_eight_reactions_directional_derivative():
+    """the 10-species family (8 reactions): initial-state dot-product test and
+    one rate-constant gradient against GPU central differences"""
+    say("adjoint NS=10")
+    c = adj_case((12, 8, 8), geometry="spheres", seed=19, nspec=10, nreact=8)
+    nsteps, spec = 4, 3
+    ctx, L = gpu_level(c)
+    s0 = perturbed_state(c)
+    theta = theta_of(c.plugin)
+    J0, g = run_objective(L, theta, s0, nsteps, 2, spec)
+    lam = L.adjoint_state_grad()
+    rng = np.random.default_rng(8)
+    d = rng.uniform(-1.0, 1.0, s0.shape) * np.abs(s0) * 1e-7
+    d[11] = 0.0
+    Jp, _ = run_objective(L, theta, s0 + d, nsteps, 2, spec, with_grad=False)
+    Jm, _ = run_objective(L, theta, s0 - d, nsteps, 2, spec, with_grad=False)
+    fd = (Jp - Jm) / 2.0
+    ad = float(np.sum(lam * d))
+    say("J %.12e  <dJ/dU0, d> %.12e  fd %.12e" % (J0, ad, fd))
+    assert abs(ad - fd) <= 1e-6 * abs(fd) + 1e-18 * abs(J0), (ad, fd)
+    p = int(np.argmax(np.abs(g[:8])))
+    h = 1e-6 * abs(theta[p])
+    tp, tm = theta.copy(), theta.copy()
+    tp[p] += h
+    tm[p] -= h
+    Jp, _ = run_objective(L, tp, s0, nsteps, 2, spec, with_grad=False)
+    Jm, _ = run_objective(L, tm, s0, nsteps, 2, spec, with_grad=False)
+    fdp = (Jp - Jm) / (2 * h)
+    say("log_A[%d]: adjoint %.10e fd %.10e" % (p, g[p], fdp))
+    assert abs(g[p] - fdp) <= 1e-5 * abs(fdp), (p, g[p], fdp)
     L.close()
     ctx.close()
 
