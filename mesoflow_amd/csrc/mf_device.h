@@ -276,6 +276,10 @@ struct PPorous : PMixture<NSv, NSv - 1> {
     static constexpr int NS = NSv;
     static constexpr int NG = NSv - 1;
     static constexpr bool kHasChemistry = true;
+    // c^a for a stoichiometric coefficient other than 1 and 2, kept out of line:
+    // inlined at every (reaction, species) site of an unrolled loop it makes the
+    // code larger than the instruction cache
+    __host__ __device__ __attribute__((noinline)) static double stoich_pow(double c, double a) { return pow(c, a); }
     // mass-action rate of reaction r with rate constant krate, added to wdot
     template <class C> MFD static void add_reaction(const Tab &T, int r, double krate, const C &cell, double *wdot) {
         double concmult = 1.0;
@@ -285,7 +289,7 @@ struct PPorous : PMixture<NSv, NSv - 1> {
             if (nu < 0.0) {
                 double c = cell(FLO + s);
                 double a = fabs(nu);
-                concmult *= (a == 1.0) ? c : ((a == 2.0) ? c * c : pow(c, a));
+                concmult *= (a == 1.0) ? c : ((a == 2.0) ? c * c : stoich_pow(c, a));
             }
         }
         double reactrate = krate * concmult;

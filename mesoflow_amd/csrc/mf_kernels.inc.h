@@ -1453,40 +1453,44 @@ __global__ __launch_bounds__(256) void k_chem_split(ChemSplitArgs c) {
             double L[NV];
 #pragma unroll
             for (int n = 0; n < NV; ++n) L[n] = lp[(int64_t)n * ln];
-            double u[NS], prev[NS];
+            double u[NS], lsp[NS], src[NS], st[NS];
 #pragma unroll
             for (int s = 0; s < NS; ++s) u[s] = S[FLO + s];
             const double hdt = kHalf * c.dt;
-            // rate constants depend on temperature only: those of the current
-            // state, and of the lag state for the first substep (later lag
-            // states carry the current temperature); the same values the
-            // source would recompute at every stage
+            // rate constants depend on temperature only: the lag state's for the
+            // first stage of the first substep, the current state's otherwise
+            // (the same values the source would recompute at every stage)
             double kS[MF_MAX_REACTIONS], kL[MF_MAX_REACTIONS];
             P::rate_constants(T, S[TEMP], kS);
             P::rate_constants(T, L[TEMP], kL);
-            for (int m = 0; m < c.nsub; ++m) {
-                double src[NS], st[NS];
-                // stage 1 on the lag state (all its components)
-                P::spec_source_k(T, m == 0 ? kL : kS, ChemCell<NS>{L, L + FLO}, src);
+            // first substep: stage 1 on the whole lag state
+            P::spec_source_k(T, kL, ChemCell<NS>{L, L + FLO}, src);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) st[s] = kOne * u[s] + hdt * (kZero + src[s]);
+            P::spec_source_k(T, kS, ChemCell<NS>{S, st}, src);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                lsp[s] = u[s];
+                u[s] = kOne * u[s] + c.dt * (kZero + src[s]);
+            }
+            // later substeps: the lag state is the current one with the species
+            // of the previous substep
+            for (int m = 1; m < c.nsub; ++m) {
+                P::spec_source_k(T, kS, ChemCell<NS>{S, lsp}, src);
 #pragma unroll
                 for (int s = 0; s < NS; ++s) st[s] = kOne * u[s] + hdt * (kZero + src[s]);
-                // stage 2 on the half-step state (current non-species components)
                 P::spec_source_k(T, kS, ChemCell<NS>{S, st}, src);
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
-                    prev[s] = u[s];
+                    lsp[s] = u[s];
                     u[s] = kOne * u[s] + c.dt * (kZero + src[s]);
                 }
-                // the state before this substep becomes the lag state
-#pragma unroll
-                for (int n = 0; n < NV; ++n) L[n] = S[n];
-#pragma unroll
-                for (int s = 0; s < NS; ++s) L[FLO + s] = prev[s];
             }
+            // the state before the last substep becomes the lag state
+#pragma unroll
+            for (int n = 0; n < NV; ++n) lp[(int64_t)n * ln] = (n >= FLO) ? lsp[n - FLO] : S[n];
 #pragma unroll
             for (int s = 0; s < NS; ++s) S[FLO + s] = u[s];
-#pragma unroll
-            for (int n = 0; n < NV; ++n) lp[(int64_t)n * ln] = L[n];
         }
         if (c.finish && S[VFRAC] >= kOne) {
             const double rho = P::get_r_from_c(T, S + FLO);
