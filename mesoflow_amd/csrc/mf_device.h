@@ -508,11 +508,12 @@ struct Num {
     // (the reference forms 1/q_k and 1/sum alpha; the value is the same to
     // rounding). Building with MF_WENO_REFERENCE_DIVISIONS restores the
     // reference's four divisions.
-    // n / d for the WENO weight sum d (positive, far inside the normal range):
-    // v_rcp_f64 estimate refined by two Newton-Raphson steps r += r (1 - d r)
-    // and one residual correction q += r (n - d q) -- the steps of the full
-    // division without the exponent scaling that only extreme operands need
-    MFD static double div_weight_sum(double n, double d) {
+    // n / d for a positive d far inside the normal range (WENO weight sums,
+    // sums of transport coefficients plus 1e-30): v_rcp_f64 estimate refined by
+    // two Newton-Raphson steps r += r (1 - d r) and one residual correction
+    // q += r (n - d q) -- the steps of the full division without the exponent
+    // scaling that only extreme operands need
+    MFD static double div_pos(double n, double d) {
 #if defined(__HIP_DEVICE_COMPILE__)
         double r = __builtin_amdgcn_rcp(d);
         double e = fma(-d, r, 1.0);
@@ -539,7 +540,7 @@ struct Num {
         const double q1 = (eps + b1) * (eps + b1);
         const double q0 = (eps + b0) * (eps + b0);
         const double w2 = q1 * q0, w1 = 6.0 * (q2 * q0), w0 = 3.0 * (q2 * q1);
-        return div_weight_sum(w2 * v2 + w1 * v1 + w0 * v0, 6.0 * (w2 + w1 + w0));
+        return div_pos(w2 * v2 + w1 * v1 + w0 * v0, 6.0 * (w2 + w1 + w0));
 #endif
     }
     MFD static double weno_plus(double s0, double s1, double s2, double s3, double s4) {
@@ -572,6 +573,8 @@ struct Num {
     }
     // compute_flux_3d.H:16-22
     MFD static double harmonic_mean(double d1, double d2) { return 2.0 * d1 * d2 / (d1 + d2 + 1e-30); }
+    // the same for non-negative coefficients, with div_pos
+    MFD static double harmonic_mean_pos(double d1, double d2) { return div_pos(2.0 * d1 * d2, d1 + d2 + 1e-30); }
 };
 
 }  // namespace mfd

@@ -437,7 +437,9 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
     extern __shared__ double lds[];
     if (step_skipped(a.ts)) return;
     const Tab &T = *a.tab;
-    const double dxs = a.dx[D];
+    // reciprocals of the spacings of the normal and transverse differences
+    // (x * (1/dx) instead of x / dx: rounding differs in the last bit)
+    const double idn = 1.0 / a.dx[D], it1 = 1.0 / (kTwo * a.dx[T1]), it2 = 1.0 / (kTwo * a.dx[T2]);
     const int full = a.flux_full;
     for (TileIter it(w); it.ok(); it.next()) {
         const int t = it.t;
@@ -516,8 +518,8 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
                         CellRef c1c{&acc, i - 1, j, k, acc.cid(i - 1, j, k)};
                         P::fluid_transport(T, c1c, v1, k1);
                     }
-                    visc = N::harmonic_mean(v0, v1);
-                    thcond = N::harmonic_mean(k0, k1);
+                    visc = N::harmonic_mean_pos(v0, v1);
+                    thcond = N::harmonic_mean_pos(k0, k1);
                 }
                 if (hyp) {
                     constexpr int p10 = T1 == 0, p11 = T1 == 1, p12 = T1 == 2;
@@ -533,17 +535,17 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
                         const int V = VELX + q;
                         vL[q] = sel_get(acc, iL, jL, kL, V, cL);
                         vR[q] = sel_get(acc, i, j, k, V, cR);
-                        g[q][0] = (vR[q] - vL[q]) / dxs;
+                        g[q][0] = (vR[q] - vL[q]) * idn;
                         double top = kHalf * (sel_get(acc, iL + p10, jL + p11, kL + p12, V, tc0) +
                                               sel_get(acc, i + p10, j + p11, k + p12, V, tc1));
                         double bot = kHalf * (sel_get(acc, iL - p10, jL - p11, kL - p12, V, tc2) +
                                               sel_get(acc, i - p10, j - p11, k - p12, V, tc3));
-                        g[q][1] = (top - bot) / (kTwo * a.dx[T1]);
+                        g[q][1] = (top - bot) * it1;
                         double frt = kHalf * (sel_get(acc, iL + p20, jL + p21, kL + p22, V, tc4) +
                                               sel_get(acc, i + p20, j + p21, k + p22, V, tc5));
                         double bck = kHalf * (sel_get(acc, iL - p20, jL - p21, kL - p22, V, tc6) +
                                               sel_get(acc, i - p20, j - p21, k - p22, V, tc7));
-                        g[q][2] = (frt - bck) / (kTwo * a.dx[T2]);
+                        g[q][2] = (frt - bck) * it2;
                     }
                     // axis -> gradient slot (GET_XDIR/GET_YDIR/GET_ZDIR, globalDefines.H:49-51)
                     constexpr int sx_ = D == 0 ? 0 : (T1 == 0 ? 1 : 2);
@@ -574,7 +576,7 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
                     de1 = t1 * (kHalf * (vL[1] + vR[1]));
                     de2 = t2 * (kHalf * (vL[2] + vR[2]));
                 }
-                const double dTdn = (sel_get(acc, i, j, k, TEMP, cR) - sel_get(acc, iL, jL, kL, TEMP, cL)) / dxs;
+                const double dTdn = (sel_get(acc, i, j, k, TEMP, cR) - sel_get(acc, iL, jL, kL, TEMP, cL)) * idn;
                 de3 = thcond * dTdn;
                 double dl[NS], dr[NS];
                 LdsCell<NC> lc{recL, &acc, iL, jL, kL, cL};
@@ -583,8 +585,8 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
                 P::spec_dcoeff(T, rc, dr);
 #pragma unroll
                 for (int s = 0; s < NS; ++s) {
-                    double Ds = N::harmonic_mean(dl[s], dr[s]);
-                    dsp[s] = Ds * (recR[NCV + s] - recL[NCV + s]) / dxs;
+                    double Ds = N::harmonic_mean_pos(dl[s], dr[s]);
+                    dsp[s] = Ds * (recR[NCV + s] - recL[NCV + s]) * idn;
                 }
             }
             const View &F = b.flux[D];
