@@ -1041,6 +1041,7 @@ int upload_tab(Ctx *c, const mf_plugin_params *plugin) {
     std::memcpy(static_cast<mf_plugin_params *>(&ht), plugin, sizeof(mf_plugin_params));
     for (int q = 0; q < MF_MAX_SPECIES; ++q) {
         ht.gm1[q] = plugin->gamma[q] - 1.0;
+        ht.igm1[q] = 1.0 / (plugin->gamma[q] - 1.0);
         ht.cv[q] = plugin->R_univ / (plugin->gamma[q] - 1.0);
         ht.cvm[q] = plugin->R_univ / (plugin->gamma[q] - 1.0) / plugin->molwt[q];
     }
@@ -1112,6 +1113,7 @@ int mf_ctx_create(const mf_plugin_params *plugin, const mf_params *params, const
     std::memcpy(static_cast<mf_plugin_params *>(&ht), plugin, sizeof(mf_plugin_params));
     for (int q = 0; q < MF_MAX_SPECIES; ++q) {
         ht.gm1[q] = plugin->gamma[q] - 1.0;
+        ht.igm1[q] = 1.0 / (plugin->gamma[q] - 1.0);
         ht.cv[q] = plugin->R_univ / (plugin->gamma[q] - 1.0);
         ht.cvm[q] = plugin->R_univ / (plugin->gamma[q] - 1.0) / plugin->molwt[q];
     }

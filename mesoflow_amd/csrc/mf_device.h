@@ -91,13 +91,35 @@ struct CellArr {
 // ---------------------------------------------------------------------------
 // Plugins: compile-time families with runtime tables (mf_plugin_params)
 // ---------------------------------------------------------------------------
+// n / d for a positive d far inside the normal range (WENO weight sums,
+// densities, heat capacities, sums of concentrations or of transport
+// coefficients plus 1e-30): v_rcp_f64 estimate refined by
+// two Newton-Raphson steps r += r (1 - d r) and one residual correction
+// q += r (n - d q) -- the steps of the full division without the exponent
+// scaling that only extreme operands need
+MFD double div_pos(double n, double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    const double q = n * r;
+    return fma(fma(-d, q, n), r, q);
+#else
+    return n / d;
+#endif
+}
+
 // Plugin table on the device: the C-ABI table plus per-species constants the
 // plugins would otherwise recompute per call (the same IEEE divisions, done once
-// on the host: cv_s = R/(gamma_s-1), gm1_s = gamma_s-1, cvm_s = R/(gamma_s-1)/molwt_s).
+// on the host: cv_s = R/(gamma_s-1), gm1_s = gamma_s-1, cvm_s = R/(gamma_s-1)/molwt_s),
+// and igm1_s = 1/(gamma_s-1) for the mixture gamma.
 struct Tab : mf_plugin_params {
     double cv[MF_MAX_SPECIES];
     double gm1[MF_MAX_SPECIES];
     double cvm[MF_MAX_SPECIES];
+    double igm1[MF_MAX_SPECIES];
 };
 
 // tests/Cartesian/ShockTube_N2/{thermo.H:19-133, transport.H, userfuncs.H, chemistry.H, externbc.H}
@@ -178,7 +200,7 @@ struct PMixture {
         double cvs = kZero;
 #pragma unroll
         for (int s = 0; s < NG; ++s) cvs += T.cv[s] * spec[s];
-        double Tm = rhoe / cvs;
+        double Tm = div_pos(rhoe, cvs);
         double p = kZero;
 #pragma unroll
         for (int s = 0; s < NG; ++s) p += spec[s] * T.R_univ * Tm;
@@ -188,10 +210,12 @@ struct PMixture {
         double sum_ci = kZero;
 #pragma unroll
         for (int s = 0; s < NG; ++s) sum_ci += spec[s];
+        // x_s / (gamma_s - 1) with one reciprocal of the total and of each gamma_s - 1
+        const double isum = div_pos(kOne, sum_ci);
         double term2 = kZero;
 #pragma unroll
-        for (int s = 0; s < NG; ++s) term2 += (spec[s] / sum_ci) / T.gm1[s];
-        return kOne + kOne / term2;
+        for (int s = 0; s < NG; ++s) term2 += (spec[s] * isum) * T.igm1[s];
+        return kOne + div_pos(kOne, term2);
     }
     MFD static double get_e_from_rpc(const Tab &T, double rho, double pres, const double *spec) {
         double temp = get_t_from_rpc(T, rho, pres, spec);
@@ -397,9 +421,10 @@ struct Num {
     // pressure and velocities used by geteulerflux (consprim_funcs.H:57-66)
     MFD static void euler_prims(const Tab &T, const double *u, double &vx, double &vy, double &vz, double &p) {
         double rho = u[RHO];
-        vx = u[RHOU] / rho;
-        vy = u[RHOV] / rho;
-        vz = u[RHOW] / rho;
+        const double irho = div_pos(kOne, rho);
+        vx = u[RHOU] * irho;
+        vy = u[RHOV] * irho;
+        vz = u[RHOW] * irho;
         double rhoe = u[RHOE] - kHalf * u[RHO] * (vx * vx + vy * vy + vz * vz);
         p = P::get_p_from_rec(T, rhoe, u[RHO], u + NCV);
     }
@@ -508,24 +533,6 @@ struct Num {
     // (the reference forms 1/q_k and 1/sum alpha; the value is the same to
     // rounding). Building with MF_WENO_REFERENCE_DIVISIONS restores the
     // reference's four divisions.
-    // n / d for a positive d far inside the normal range (WENO weight sums,
-    // sums of transport coefficients plus 1e-30): v_rcp_f64 estimate refined by
-    // two Newton-Raphson steps r += r (1 - d r) and one residual correction
-    // q += r (n - d q) -- the steps of the full division without the exponent
-    // scaling that only extreme operands need
-    MFD static double div_pos(double n, double d) {
-#if defined(__HIP_DEVICE_COMPILE__)
-        double r = __builtin_amdgcn_rcp(d);
-        double e = fma(-d, r, 1.0);
-        r = fma(r, e, r);
-        e = fma(-d, r, 1.0);
-        r = fma(r, e, r);
-        const double q = n * r;
-        return fma(fma(-d, q, n), r, q);
-#else
-        return n / d;
-#endif
-    }
     MFD static double weno_combine(double b2, double b1, double b0, double v2, double v1, double v0) {
         const double eps = 1e-6;
 #ifdef MF_WENO_REFERENCE_DIVISIONS

@@ -56,6 +56,7 @@ static Tab make_tab() {
     t.out_pres = 1.0e5;
     for (int q = 0; q < MF_MAX_SPECIES; ++q) {
         t.gm1[q] = t.gamma[q] - 1.0;
+        t.igm1[q] = 1.0 / (t.gamma[q] - 1.0);
         t.cv[q] = t.R_univ / (t.gamma[q] - 1.0);
         t.cvm[q] = t.R_univ / (t.gamma[q] - 1.0) / t.molwt[q];
     }
