@@ -1341,24 +1341,26 @@ __global__ __launch_bounds__(128) void k_cutcells(CutArgs a) {
             const int l = nb % 5 - 2, m = (nb / 5) % 5 - 2, n = nb / 25 - 2;
             const double *c = s.p + s.off(i + l, j + m, k + n);
             const int64_t ns = s.ns;
+            // one reciprocal per weight, multiplied into each value (the
+            // reference divides every value by the squared distance)
             if (!vel0 && !(l == 0 && m == 0 && n == 0)) {
                 double a0 = l * dnd[0], a1 = m * dnd[1], a2 = n * dnd[2];
-                double w2 = a0 * a0 + a1 * a1 + a2 * a2 + kTiny2;
-                su0 += c[(int64_t)VELX * ns] / w2;
-                su1 += c[(int64_t)VELY * ns] / w2;
-                su2 += c[(int64_t)VELZ * ns] / w2;
-                swv += 1.0 / w2;
+                const double iw = div_pos(1.0, a0 * a0 + a1 * a1 + a2 * a2 + kTiny2);
+                su0 += c[(int64_t)VELX * ns] * iw;
+                su1 += c[(int64_t)VELY * ns] * iw;
+                su2 += c[(int64_t)VELZ * ns] * iw;
+                swv += iw;
             }
             double a0 = l * dnd[0] - xnd[0], a1 = m * dnd[1] - xnd[1], a2 = n * dnd[2] - xnd[2];
-            double w2 = a0 * a0 + a1 * a1 + a2 * a2 + kTiny2;
-            sp += c[(int64_t)PRES * ns] / w2;
-            if (doT) sT += c[(int64_t)TEMP * ns] / w2;
-            sR += c[(int64_t)DENS * ns] / w2;
+            const double iw = div_pos(1.0, a0 * a0 + a1 * a1 + a2 * a2 + kTiny2);
+            sp += c[(int64_t)PRES * ns] * iw;
+            if (doT) sT += c[(int64_t)TEMP * ns] * iw;
+            sR += c[(int64_t)DENS * ns] * iw;
             if (doS) {
 #pragma unroll
-                for (int q2 = 0; q2 < NS; ++q2) sS[q2] += c[(int64_t)(FLO + q2) * ns] / w2;
+                for (int q2 = 0; q2 < NS; ++q2) sS[q2] += c[(int64_t)(FLO + q2) * ns] * iw;
             }
-            swn += 1.0 / w2;
+            swn += iw;
         }
     }
     // Dirichlet (u = 0) IDW of the velocities, :98-152
