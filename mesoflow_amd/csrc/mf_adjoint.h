@@ -32,9 +32,10 @@ MFD void weno_plus_rev(const double *s, double gbar, double *sb) {
     const double Q2 = e2 * e2, Q1 = e1 * e1, Q0 = e0 * e0;
     const double w2 = Q1 * Q0, w1 = 6.0 * (Q2 * Q0), w0 = 3.0 * (Q2 * Q1);
     const double den = 6.0 * (w2 + w1 + w0);
-    const double val = (w2 * v2 + w1 * v1 + w0 * v0) / den;
-    const double nb = gbar / den;
-    const double db = -gbar * val / den;
+    const double iden = div_pos(1.0, den);  // one reciprocal of the positive weight sum
+    const double val = (w2 * v2 + w1 * v1 + w0 * v0) * iden;
+    const double nb = gbar * iden;
+    const double db = -gbar * val * iden;
     const double w2b = nb * v2 + 6.0 * db, w1b = nb * v1 + 6.0 * db, w0b = nb * v0 + 6.0 * db;
     const double v2b = nb * w2, v1b = nb * w1, v0b = nb * w0;
     const double Q2b = w1b * 6.0 * Q0 + w0b * 3.0 * Q1;
