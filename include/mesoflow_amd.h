@@ -290,6 +290,12 @@ int mf_adjoint_state_grad(void *level, double *out, int is_device);
 
 /* Plain HBM copy of n doubles on the context stream (bandwidth calibration). */
 int mf_util_copy(void *ctx, double *dst, const double *src, int64_t n);
+/* Check of this build's positive-denominator division (no counterpart in the
+ * reference): n seeded operand pairs, numerators of either sign with binary
+ * exponents in [-40, 40], denominators in [2^lo_exp, 2^(hi_exp+1)); counts[3]
+ * receives how many quotients equal the IEEE division, differ by one ulp, or
+ * by more. */
+int mf_util_div_check(void *ctx, int64_t n, int lo_exp, int hi_exp, int64_t *counts);
 
 #ifdef __cplusplus
 }

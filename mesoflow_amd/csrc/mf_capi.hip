@@ -1818,6 +1818,53 @@ int mf_util_copy(void *ctx, double *dst, const double *src, int64_t n) {
     return MF_OK;
 }
 
+// div_pos against the IEEE division over n seeded operand pairs: numerators
+// of either sign with exponents in [-40, 40], positive denominators with
+// exponents in [lo_exp, hi_exp]; counts[0] equal, [1] one ulp apart, [2] more
+__global__ void k_div_check(int64_t n, int lo_exp, int hi_exp, unsigned long long *counts) {
+    unsigned long long c0 = 0, c1 = 0, c2 = 0;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        // splitmix64 of the index
+        uint64_t z = (uint64_t)q * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+        auto next = [&]() {
+            z += 0x9E3779B97F4A7C15ull;
+            uint64_t x = z;
+            x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+            x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+            return x ^ (x >> 31);
+        };
+        const uint64_t r1 = next(), r2 = next(), r3 = next(), r4 = next();
+        const double m1 = 1.0 + (double)(r1 >> 11) * 0x1.0p-53, m2 = 1.0 + (double)(r2 >> 11) * 0x1.0p-53;
+        const int e1 = (int)(r3 % 81) - 40, e2 = lo_exp + (int)(r4 % (uint64_t)(hi_exp - lo_exp + 1));
+        const double num = ((r3 >> 32) & 1 ? -1.0 : 1.0) * ldexp(m1, e1);
+        const double den = ldexp(m2, e2);
+        const double a = num / den, b = div_pos(num, den);
+        const int64_t ia = __double_as_longlong(a), ib = __double_as_longlong(b);
+        const int64_t d = ia > ib ? ia - ib : ib - ia;
+        if (d == 0) ++c0;
+        else if (d == 1) ++c1;
+        else ++c2;
+    }
+    atomicAdd(counts + 0, c0);
+    atomicAdd(counts + 1, c1);
+    atomicAdd(counts + 2, c2);
+}
+
+int mf_util_div_check(void *ctx, int64_t n, int lo_exp, int hi_exp, int64_t *counts) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c || n < 0 || !counts || lo_exp > hi_exp || lo_exp < -900 || hi_exp > 900) return MF_ERR_BAD_ARG;
+    unsigned long long *d = nullptr;
+    HIPCHK(hipMalloc(&d, 3 * sizeof(unsigned long long)));
+    HIPCHK(hipMemsetAsync(d, 0, 3 * sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_div_check, dim3(2048), dim3(256), 0, c->stream, n, lo_exp, hi_exp, d);
+    unsigned long long h[3];
+    HIPCHK(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    hipFree(d);
+    for (int q = 0; q < 3; ++q) counts[q] = (int64_t)h[q];
+    return MF_OK;
+}
+
 int mf_adjoint_forward(void *level, int nsteps, int ncheck, int spec, double *J_out) {
     Level *L = (Level *)level;
     if (!L || nsteps < 1 || ncheck < 1) return MF_ERR_BAD_ARG;

@@ -54,6 +54,7 @@ SIGNATURES = {
     "mf_adjoint_backward": ([C.c_void_p, C.c_void_p, _P(C.c_double)], C.c_int),
     "mf_adjoint_state_grad": ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),
     "mf_util_copy": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64], C.c_int),
+    "mf_util_div_check": ([C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_void_p], C.c_int),
 }
 
 

@@ -4,7 +4,8 @@ not depend on the domain size:
 * window parity (tests/window.py): windows of the full-size GPU result match
   the oracle run on the windows grown by the dependence halo -- the 256^3
   level of 512 boxes with 5 species (2 steps) and the 512^3 level with 10
-  species and 8 reactions (1 step), windows at the inflow, at the outflow and
+  species and 8 reactions (1 step), the 256^3 level of about 1200 uneven
+  boxes on a GRF solid (1 step); windows at the inflow, at the outflow and
   inside, each block given a seeded perturbation;
 * launch-mode invariance: the 256^3 state after two adaptive-dt steps is the
   same for the persistent batched launches, the per-box launches (the
@@ -74,6 +75,16 @@ def test_porous512_ns10_window_parity():
     c = cases.porous_reactor(n_cell=(512, 512, 512), nspec=10, nreact=8, geometry="grf", porosity=0.45, corr_len=8.0,
                              order_hyp=5, cfl=0.3, seed=20261207, max_grid_size=64)
     _window_parity(c, 1, 16, seed=12)
+
+
+@pytest.mark.timeout(600)
+def test_uneven256_grf_window_parity():
+    """the single-level stand-in of the many-box configuration: about 1200
+    uneven boxes (edges 8..64), GRF solid at porosity 0.5"""
+    boxes = cases.random_boxes((256, 256, 256), 8, 64, seed=7, p_stop=0.5)
+    c = cases.porous_reactor(n_cell=(256, 256, 256), nspec=5, nreact=2, geometry="grf", porosity=0.5, corr_len=8.0,
+                             order_hyp=5, cfl=0.3, seed=20261205, boxes=boxes)
+    _window_parity(c, 1, 16, seed=13)
 
 
 @pytest.mark.timeout(600)
