@@ -91,6 +91,11 @@ struct CellArr {
 // ---------------------------------------------------------------------------
 // Plugins: compile-time families with runtime tables (mf_plugin_params)
 // ---------------------------------------------------------------------------
+// std::max / std::min of the reference (a NaN first argument propagates, a
+// NaN second argument is dropped -- fmax/fmin drop both)
+MFD double std_max(double a, double b) { return (a < b) ? b : a; }
+MFD double std_min(double a, double b) { return (b < a) ? b : a; }
+
 // n / d for a positive d far inside the normal range (WENO weight sums,
 // densities, heat capacities, sums of concentrations or of transport
 // coefficients plus 1e-30): v_rcp_f64 estimate refined by
@@ -454,10 +459,10 @@ struct Num {
         double presR = P::get_p_from_rec(T, rhoeR, ur[RHO], ul + NCV);
         double velnL = ul[RHOU + D] / ul[RHO];
         double velnR = ur[RHOU + D] / ur[RHO];
-        double veln = fmax(fabs(velnL), fabs(velnR));
+        double veln = std_max(fabs(velnL), fabs(velnR));
         double gamaL = P::get_gama_from_rpc(T, rhoL, presL, ul + NCV);
         double gamaR = P::get_gama_from_rpc(T, rhoR, presR, ur + NCV);
-        double ahalf = fmax(sqrt(gamaL * presL / rhoL), sqrt(gamaR * presR / rhoR));
+        double ahalf = std_max(sqrt(gamaL * presL / rhoL), sqrt(gamaR * presR / rhoR));
         return ahalf + veln;
     }
     // flux_schemes.H:105-141
@@ -494,19 +499,19 @@ struct Num {
         double HR = gamaR / (gamaR - kOne) * (pr[4] / pr[0]) + kHalf * (pr[1] * pr[1] + pr[2] * pr[2] + pr[3] * pr[3]);
         double rho_half = kHalf * (pl[0] + pr[0]);
         double astar2 = kTwo * (gamaL - kOne) / (gamaL + kOne) * HL;
-        double ahatL = astar2 / fmax(sqrt(astar2), fabs(vL));
+        double ahatL = astar2 / std_max(sqrt(astar2), fabs(vL));
         astar2 = kTwo * (gamaR - kOne) / (gamaR + kOne) * HR;
-        double ahatR = astar2 / fmax(sqrt(astar2), fabs(vR));
-        double a_half = fmin(ahatL, ahatR);
+        double ahatR = astar2 / std_max(sqrt(astar2), fabs(vR));
+        double a_half = std_min(ahatL, ahatR);
         double ML = vL / a_half, MR = vR / a_half;
         double MLplus = split_mach(ML, beta, false);
         double MRminus = split_mach(MR, beta, true);
         double M_half = MLplus + MRminus;
         double M_bar2 = kHalf * (vL * vL + vR * vR) / (a_half * a_half);
-        double Mo2 = fmin(kOne, fmax(M_bar2, Minf2));
+        double Mo2 = std_min(kOne, std_max(M_bar2, Minf2));
         double Mo = sqrt(Mo2);
         double fa_Mo = Mo * (kTwo - Mo);
-        M_half = M_half - (Kp / fa_Mo) * fmax(kOne - sigma * M_bar2, kZero) * (pr[4] - pl[4]) / (rho_half * a_half * a_half);
+        M_half = M_half - (Kp / fa_Mo) * std_max(kOne - sigma * M_bar2, kZero) * (pr[4] - pl[4]) / (rho_half * a_half * a_half);
         double mod_M_half = fabs(M_half);
         if (mod_M_half <= delta) mod_M_half = (M_half * M_half + delta * delta) / (kTwo * delta);
 #pragma unroll

@@ -317,10 +317,10 @@ MFD void face_flux(const Tab &T, const Acc &acc, const StageArgs &a, int i, int 
             double rhoeR = u0[RHOE] - kHalf * (u0[RHOU] * u0[RHOU] + u0[RHOV] * u0[RHOV] + u0[RHOW] * u0[RHOW]) / u0[RHO];
             double presL = P::get_p_from_rec(T, rhoeL, um1[RHO], um1 + NCV);
             double presR = P::get_p_from_rec(T, rhoeR, u0[RHO], u0 + NCV);
-            double veln = fmax(fabs(um1[RHOU + D] / um1[RHO]), fabs(u0[RHOU + D] / u0[RHO]));
+            double veln = std_max(fabs(um1[RHOU + D] / um1[RHO]), fabs(u0[RHOU + D] / u0[RHO]));
             double gamaL = P::get_gama_from_rpc(T, rhoL, presL, um1 + NCV);
             double gamaR = P::get_gama_from_rpc(T, rhoR, presR, u0 + NCV);
-            double ahalf = fmax(sqrt(gamaL * presL / rhoL), sqrt(gamaR * presR / rhoR));
+            double ahalf = std_max(sqrt(gamaL * presL / rhoL), sqrt(gamaR * presR / rhoR));
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
                 double fm2 = N::template euler_comp<D>(c, um2, pv[0][0], pv[0][1], pv[0][2], pv[0][3]);
@@ -617,9 +617,9 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
                     double rhoeR = ur[RHOE] - kHalf * (ur[RHOU] * ur[RHOU] + ur[RHOV] * ur[RHOV] + ur[RHOW] * ur[RHOW]) / ur[RHO];
                     double presL = P::get_p_from_rec(T, rhoeL, ul[RHO], ul + NCV);
                     double presR = P::get_p_from_rec(T, rhoeR, ur[RHO], ul + NCV);
-                    double veln = fmax(fabs(recL[FVX + D]), fabs(recR[FVX + D]));
+                    double veln = std_max(fabs(recL[FVX + D]), fabs(recR[FVX + D]));
                     double gamaL = recL[FG], gamaR = recR[FG];
-                    double ahalf = fmax(sqrt(gamaL * presL / ul[RHO]), sqrt(gamaR * presR / ur[RHO]));
+                    double ahalf = std_max(sqrt(gamaL * presL / ul[RHO]), sqrt(gamaR * presR / ur[RHO]));
                     lam = ahalf + veln;
                 }
 #pragma unroll
