@@ -107,3 +107,15 @@ def test_residual_norms_match():
     operand = np.sqrt((ref.reshape(ref.shape[0], -1) ** 2).sum(axis=1)) / dt
     assert np.all(np.abs(r - g) <= 1e-11 * operand + 1e-12 * np.abs(r)), (r, g)
     assert np.all(r[:5] == 0.0) and np.all(g[:5] == 0.0)   # conserved: chemistry leaves them unchanged
+
+
+def test_porous_tiny_boxes():
+    """boxes with edges of 1 to 3 cells (435 boxes): tiles cut to the box, face
+    sweeps whose stencil reaches three boxes away, cut cells of other boxes
+    read raw"""
+    c = cases.porous_reactor(n_cell=(14, 9, 7), nspec=5, nreact=2, geometry="spheres", porosity=0.6, rmin=2, rmax=3,
+                             order_hyp=5, seed=29)
+    c.boxes = cases.random_boxes(c.n_cell, 1, 3, seed=5, p_stop=0.3)
+    assert len(c.boxes) > 100
+    _check(c, 3, mode=abi.MODE_BATCHED)
+    _check(c, 2, mode=abi.MODE_PER_BOX)
