@@ -259,6 +259,38 @@ int mf_level_info(void *level, double *time, double *dt, int *istep, double *res
 /* Overrides the level time/dt history (restart). */
 int mf_level_set_time(void *level, double time, double dt_prev, int istep);
 
+/* Box list, geometry and component count of a level (boxes may be NULL to
+ * query the count; cap is the capacity of boxes). */
+int mf_level_boxes(void *level, mf_box *boxes, int cap, int *nboxes, mf_geom *geom, int *nv);
+
+/* Output of a level (Source/mflo_plot.cpp). varnames: nv names
+ * (Source/mflo.cpp:34-53 order), or NULL for rho, rhou, rhov, rhow, rhoe,
+ * density, velx, vely, velz, pressure, temperature, volfrac, S0, S1, ..
+ * WritePlotFile (:45-56, amrex::WriteMultiLevelPlotfile): <dir>/Header
+ * (HyperCLaw-V1.1) and <dir>/Level_0/Cell_H, Cell_D_00000, one FAB per box
+ * of the valid cells, time and step from the level. */
+int mf_level_write_plotfile(void *level, const char *dir, const char *const *varnames);
+/* WriteCheckpointFile (:57-145): <dir>/Header ("Checkpoint file for mflo",
+ * finest_level 0, istep, dt of the last step, t_new at precision 17, the
+ * BoxArray) and the state as <dir>/Level_0/<varnames[0]>_H, _D_00000. */
+int mf_level_write_checkpoint(void *level, const char *dir, const char *const *varnames);
+/* ReadCheckpointFile (:147-239) into a level created with the same boxes
+ * (MF_ERR_BAD_ARG if the Header's BoxArray or the FABs differ,
+ * MF_ERR_UNSUPPORTED for finest_level > 0): the state (mf_level_set_state)
+ * and the time, dt and step (mf_level_set_time). */
+int mf_level_read_checkpoint(void *level, const char *dir, const char *const *varnames);
+
+/* The same files from host data (no device work): state is
+ * [nv][nz][ny][nx] over geom's domain, only the cells of the boxes are
+ * written or read; boxes must lie inside the domain (MF_ERR_BAD_ARG). For a
+ * host that holds the state itself, e.g. after mf_level_get_state. */
+int mf_write_plotfile(const char *dir, const mf_geom *geom, const mf_box *boxes, int nboxes, int nv,
+                      const double *state, double time, int istep, const char *const *varnames);
+int mf_write_checkpoint(const char *dir, const mf_geom *geom, const mf_box *boxes, int nboxes, int nv,
+                        const double *state, double time, double dt, int istep, const char *const *varnames);
+int mf_read_checkpoint(const char *dir, const mf_geom *geom, const mf_box *boxes, int nboxes, int nv, double *state,
+                       double *time, double *dt, int *istep, const char *const *varnames);
+
 /* Tile shape of a kernel class: 0, 1, 2 the LDS-staged WENO5 face sweeps of
  * direction x, y, z; 3 the cell tiles (residual + RK3 combine, dt estimate,
  * face sweeps of the other schemes). */

@@ -1682,6 +1682,20 @@ int mf_level_info(void *level, double *time, double *dt, int *istep, double *res
     return MF_OK;
 }
 
+int mf_level_boxes(void *level, mf_box *boxes, int cap, int *nboxes, mf_geom *geom, int *nv) {
+    Level *L = (Level *)level;
+    if (!L) return MF_ERR_BAD_ARG;
+    const int nb = (int)L->boxes.size();
+    if (nboxes) *nboxes = nb;
+    if (geom) *geom = L->c->geom;
+    if (nv) *nv = L->c->nv;
+    if (boxes) {
+        if (cap < nb) return MF_ERR_BAD_ARG;
+        std::memcpy(boxes, L->boxes.data(), sizeof(mf_box) * nb);
+    }
+    return MF_OK;
+}
+
 int mf_level_set_time(void *level, double time, double dt_prev, int istep) {
     Level *L = (Level *)level;
     if (!L) return MF_ERR_BAD_ARG;
@@ -1955,3 +1969,15 @@ int mf_adjoint_state_grad(void *level, double *out, int is_device) {
 }
 
 }  // extern "C"
+
+// dt of the level's last step (the dt[lev] that a checkpoint records and the
+// next ComputeDt caps against; 1e100 before the first step), for mf_io.hip
+int mf_level_dt_prev(void *level, double *dt_prev) {
+    Level *L = (Level *)level;
+    if (!L || !dt_prev) return MF_ERR_BAD_ARG;
+    TimeState h;
+    int st = read_ts(L, &h);
+    if (st) return st;
+    *dt_prev = h.dt_prev;
+    return MF_OK;
+}

@@ -47,6 +47,16 @@ SIGNATURES = {
                               C.c_int),
     "mf_level_info": ([C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_int), C.c_void_p], C.c_int),
     "mf_level_set_time": ([C.c_void_p, C.c_double, C.c_double, C.c_int], C.c_int),
+    "mf_level_boxes": ([C.c_void_p, _P(abi.Box), C.c_int, _P(C.c_int), _P(abi.Geom), _P(C.c_int)], C.c_int),
+    "mf_level_write_plotfile": ([C.c_void_p, C.c_char_p, _P(C.c_char_p)], C.c_int),
+    "mf_level_write_checkpoint": ([C.c_void_p, C.c_char_p, _P(C.c_char_p)], C.c_int),
+    "mf_level_read_checkpoint": ([C.c_void_p, C.c_char_p, _P(C.c_char_p)], C.c_int),
+    "mf_write_plotfile": ([C.c_char_p, _P(abi.Geom), _P(abi.Box), C.c_int, C.c_int, C.c_void_p, C.c_double, C.c_int,
+                           _P(C.c_char_p)], C.c_int),
+    "mf_write_checkpoint": ([C.c_char_p, _P(abi.Geom), _P(abi.Box), C.c_int, C.c_int, C.c_void_p, C.c_double,
+                             C.c_double, C.c_int, _P(C.c_char_p)], C.c_int),
+    "mf_read_checkpoint": ([C.c_char_p, _P(abi.Geom), _P(abi.Box), C.c_int, C.c_int, C.c_void_p, _P(C.c_double),
+                            _P(C.c_double), _P(C.c_int), _P(C.c_char_p)], C.c_int),
     "mf_level_set_tile": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     "mf_level_get_tile": ([C.c_void_p, C.c_int, _P(C.c_int), _P(C.c_int), _P(C.c_int)], C.c_int),
     "mf_level_autotune": ([C.c_void_p], C.c_int),
@@ -228,6 +238,36 @@ class Level:
 
     def set_time(self, t, dt_prev, istep):
         check(lib().mf_level_set_time(self.h, float(t), float(dt_prev), int(istep)), "mf_level_set_time")
+
+    def boxes(self):
+        nb, nv = C.c_int(), C.c_int()
+        g = abi.Geom()
+        check(lib().mf_level_boxes(self.h, None, 0, C.byref(nb), C.byref(g), C.byref(nv)), "mf_level_boxes")
+        arr = (abi.Box * nb.value)()
+        check(lib().mf_level_boxes(self.h, arr, nb.value, C.byref(nb), None, None), "mf_level_boxes")
+        return [(tuple(b.lo), tuple(b.hi)) for b in arr]
+
+    @staticmethod
+    def _names(varnames):
+        if varnames is None:
+            return None
+        arr = (C.c_char_p * len(varnames))(*[v.encode() for v in varnames])
+        return arr
+
+    def write_plotfile(self, path, varnames=None):
+        """WritePlotFile (Source/mflo_plot.cpp:45-56)"""
+        check(lib().mf_level_write_plotfile(self.h, os.fsencode(path), self._names(varnames)),
+              "mf_level_write_plotfile")
+
+    def write_checkpoint(self, path, varnames=None):
+        """WriteCheckpointFile (Source/mflo_plot.cpp:57-145)"""
+        check(lib().mf_level_write_checkpoint(self.h, os.fsencode(path), self._names(varnames)),
+              "mf_level_write_checkpoint")
+
+    def read_checkpoint(self, path, varnames=None):
+        """ReadCheckpointFile (Source/mflo_plot.cpp:147-239) into this level"""
+        check(lib().mf_level_read_checkpoint(self.h, os.fsencode(path), self._names(varnames)),
+              "mf_level_read_checkpoint")
 
     TILE_CLASSES = ("faces_x", "faces_y", "faces_z", "cells")
 
