@@ -170,6 +170,16 @@ View from_array4(const mf_array4 &a) {
 
 int div_up(int a, int b) { return (a + b - 1) / b; }
 
+// the view holds bx (grown by one cell at the high end of direction hi_dir, if 0..2) and nv components
+bool covers(const mf_array4 &a, const mf_box &bx, int nv, int hi_dir = -1) {
+    if (!a.p || a.ncomp < nv) return false;
+    for (int d = 0; d < 3; ++d) {
+        const int hi = bx.hi[d] + (d == hi_dir ? 1 : 0);
+        if (bx.lo[d] < a.begin[d] || hi >= a.end[d]) return false;
+    }
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // plugin-independent kernels
 // ---------------------------------------------------------------------------
@@ -1225,7 +1235,7 @@ int mf_box_rhs(void *ctx, mf_box bx, mf_array4 sborder, mf_array4 dsdt, const mf
     if (flux3) {
         full = 1;
         for (int d = 0; d < 3; ++d) {
-            if (!flux3[d].p || flux3[d].ncomp < c->nv) return MF_ERR_BAD_ARG;
+            if (!covers(flux3[d], bx, c->nv, d)) return MF_ERR_BAD_ARG;
             w.one.flux[d] = from_array4(flux3[d]);
         }
     } else {
@@ -1282,6 +1292,7 @@ int mf_box_rk3_combine(void *ctx, int stage, mf_box bx, mf_array4 s_new, mf_arra
     if (!c || stage < 1 || stage > 3 || !s_new.p || !src.p || !dsdt.p) return MF_ERR_BAD_ARG;
     const int ex = bx.hi[0] - bx.lo[0] + 1, ey = bx.hi[1] - bx.lo[1] + 1, ez = bx.hi[2] - bx.lo[2] + 1;
     if (ex <= 0 || ey <= 0 || ez <= 0) return MF_ERR_BAD_ARG;
+    if (!covers(s_new, bx, c->nv) || !covers(src, bx, c->nv) || !covers(dsdt, bx, c->nv)) return MF_ERR_BAD_ARG;
     const int64_t tot = (int64_t)ex * ey * ez * c->nv;
     int grid = (int)std::min<int64_t>((tot + 255) / 256, 65535);
     hipLaunchKernelGGL(k_combine, dim3(grid), dim3(256), 0, c->stream, stage, from_array4(s_new), from_array4(src),
