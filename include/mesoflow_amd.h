@@ -190,6 +190,46 @@ int mf_box_rk3_combine(void *ctx, int stage, mf_box bx, mf_array4 s_new,
                        mf_array4 src, mf_array4 dsdt, double dt);
 
 /* ------------------------------------------------------------------------ */
+/* Two-level coupling: the AMReX operations with which mesoflow couples a     */
+/* fine level (refinement ratio r) to its coarse level, as per-box device     */
+/* operations over all nv components of the context's state.                 */
+/* ------------------------------------------------------------------------ */
+
+/* amrex::average_down (AverageDownTo, Source/mflo_gridwork.cpp:125-130) over
+ * the coarse box cbx: crse = (sum of the r^3 fine cells) * 1/r^3. fine must
+ * hold refine(cbx). */
+int mf_box_average_down(void *ctx, mf_box cbx, mf_array4 fine, mf_array4 crse, int ratio);
+
+/* The data FillPatch reads at a time (GetData, Source/mflo.cpp:310-330, with
+ * FillPatchSingleLevel's time interpolation): within 1e-3 (t_new - t_old) of
+ * t_new the new state, of t_old the old state, else
+ * ((t_new-time)/(t_new-t_old)) s_old + ((time-t_old)/(t_new-t_old)) s_new. */
+int mf_box_time_interp(void *ctx, mf_box bx, mf_array4 dst, mf_array4 s_old, mf_array4 s_new, double t_old,
+                       double t_new, double time);
+
+/* cell_cons_interp, the mapper of FillPatchTwoLevels and FillCoarsePatch
+ * (Source/mflo_gridwork.cpp:151-165, 170-193), over the fine box fbx:
+ * MC-limited slopes of each coarse cell, scaled so that no fine value leaves
+ * the range of the 27 coarse neighbours; conservative (the mean of the r^3
+ * children is the coarse value). crse must hold coarsen(fbx) grown by 1. */
+int mf_box_interp_cons(void *ctx, mf_box fbx, mf_array4 crse, mf_array4 fine, int ratio);
+
+/* FluxRegister on one face box (coarse face indices; along dir lo == hi is the
+ * face index) of the boundary of a coarsened fine box; reg is a face-centred
+ * view of it.
+ * CrseInit(flux, dir, .., scale, ADD) (Source/mflo_evolve.cpp:655-667):
+ *   reg += scale * cflux on the face box. */
+int mf_fluxreg_crse_add(void *ctx, int dir, mf_box face, mf_array4 reg, mf_array4 cflux, double scale);
+/* FineAdd(flux, dir, .., scale) (:668-679): reg += scale * f for each of the
+ * r^2 fine faces on every coarse face. */
+int mf_fluxreg_fine_add(void *ctx, int dir, mf_box face, mf_array4 reg, mf_array4 fflux, int ratio, double scale);
+/* Reflux(S, scale, ..) (:243-246): the coarse cell outside the fine region
+ * next to each face (below it for side 0, the fine box's low side; above it
+ * for side 1) gets S += (-scale for side 0, +scale for side 1) * reg / vol. */
+int mf_fluxreg_reflux(void *ctx, int dir, int side, mf_box face, mf_array4 state, mf_array4 reg, double scale,
+                      double vol);
+
+/* ------------------------------------------------------------------------ */
 /* Level forms: one AMR level of boxes stored as one domain-sized array with  */
 /* MF_NGROW ghost layers. Boxes must tile the domain (level 0).               */
 /* ------------------------------------------------------------------------ */

@@ -1,0 +1,44 @@
+"""Two-level coupling operators on device fabs (mf_box_average_down,
+mf_box_time_interp, mf_box_interp_cons, mf_fluxreg_*): the AMReX calls of
+Source/mflo_gridwork.cpp:125-193, Source/mflo.cpp:310-330 and
+Source/mflo_evolve.cpp:232-248, 655-681. A fab is (tensor [nc][nz][ny][nx]
+on the device, lo (x, y, z) of its first cell or face). Work is enqueued on
+the context's stream; Context.sync() waits for it."""
+from . import abi, runtime
+
+
+def _a4(fab):
+    t, lo = fab
+    assert t.is_contiguous() and t.is_cuda
+    nc, nz, ny, nx = t.shape
+    return abi.make_array4(t.data_ptr(), lo, (lo[0] + nx, lo[1] + ny, lo[2] + nz), nc)
+
+
+def average_down(ctx, cbx, fine, crse, ratio):
+    runtime.check(runtime.lib().mf_box_average_down(ctx.h, abi.make_box(*cbx), _a4(fine), _a4(crse), ratio),
+                  "mf_box_average_down")
+
+
+def time_interp(ctx, bx, dst, s_old, s_new, t_old, t_new, time):
+    runtime.check(runtime.lib().mf_box_time_interp(ctx.h, abi.make_box(*bx), _a4(dst), _a4(s_old), _a4(s_new),
+                                                   float(t_old), float(t_new), float(time)), "mf_box_time_interp")
+
+
+def interp_cons(ctx, fbx, crse, fine, ratio):
+    runtime.check(runtime.lib().mf_box_interp_cons(ctx.h, abi.make_box(*fbx), _a4(crse), _a4(fine), ratio),
+                  "mf_box_interp_cons")
+
+
+def fluxreg_crse_add(ctx, d, face, reg, cflux, scale):
+    runtime.check(runtime.lib().mf_fluxreg_crse_add(ctx.h, d, abi.make_box(*face), _a4(reg), _a4(cflux),
+                                                    float(scale)), "mf_fluxreg_crse_add")
+
+
+def fluxreg_fine_add(ctx, d, face, reg, fflux, ratio, scale):
+    runtime.check(runtime.lib().mf_fluxreg_fine_add(ctx.h, d, abi.make_box(*face), _a4(reg), _a4(fflux), ratio,
+                                                    float(scale)), "mf_fluxreg_fine_add")
+
+
+def fluxreg_reflux(ctx, d, side, face, state, reg, scale, vol):
+    runtime.check(runtime.lib().mf_fluxreg_reflux(ctx.h, d, side, abi.make_box(*face), _a4(state), _a4(reg),
+                                                  float(scale), float(vol)), "mf_fluxreg_reflux")

@@ -1,0 +1,313 @@
+// mf_amr.hip -- two-level coupling operators on device views.
+//
+// The AMReX calls with which mesoflow couples a fine level to its coarse
+// level, as per-box device operations of a context (all nv components):
+//   average_down            AverageDownTo (Source/mflo_gridwork.cpp:125-130)
+//   time interpolation      GetData + FillPatchSingleLevel (Source/mflo.cpp:310-330,
+//                           Source/mflo_gridwork.cpp:147-165)
+//   cell_cons_interp        the mapper of FillPatchTwoLevels / FillCoarsePatch
+//                           (Source/mflo_gridwork.cpp:151-165, 170-193)
+//   FluxRegister            CrseInit / FineAdd (Source/mflo_evolve.cpp:655-681),
+//                           Reflux (Source/mflo_evolve.cpp:243-246)
+// One thread per output value (cell or face, component); every product that
+// feeds an add is rounded on its own (no FMA), so the results equal the CPU
+// restatement's bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "mf_args.h"
+
+using namespace mfd;
+
+int mf_ctx_stream(void *ctx, hipStream_t *stream, int *nv);  // mf_capi.hip
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxGrid = 8192;
+
+// the product rounded on its own: the empty asm hides it from FMA contraction
+__device__ __forceinline__ double rn(double p) {
+    asm volatile("" : "+v"(p));
+    return p;
+}
+
+__device__ __forceinline__ int floor_div(int a, int r) { return (a >= 0) ? a / r : -((-a + r - 1) / r); }
+
+__device__ __forceinline__ double mc_slope(double um, double u0, double up) {
+    const double dc = 0.5 * (up - um);
+    const double df = 2.0 * (up - u0);
+    const double db = 2.0 * (u0 - um);
+    const double lim = (df * db >= 0.0) ? fmin(fabs(df), fabs(db)) : 0.0;
+    return copysign(1.0, dc) * fmin(lim, fabs(dc));
+}
+
+struct Ext {  // a box as origin + extents, and a flat index -> (i, j, k, n)
+    int lo[3], e[3];
+    int64_t cells;
+    __device__ void at(int64_t q, int &i, int &j, int &k, int &n) const {
+        int64_t r = q;
+        i = lo[0] + (int)(r % e[0]);
+        r /= e[0];
+        j = lo[1] + (int)(r % e[1]);
+        r /= e[1];
+        k = lo[2] + (int)(r % e[2]);
+        n = (int)(r / e[2]);
+    }
+};
+
+Ext ext_of(const mf_box &b) {
+    Ext x;
+    x.cells = 1;
+    for (int d = 0; d < 3; ++d) {
+        x.lo[d] = b.lo[d];
+        x.e[d] = b.hi[d] - b.lo[d] + 1;
+        x.cells *= x.e[d];
+    }
+    return x;
+}
+
+View view_of(const mf_array4 &a) {
+    View v;
+    v.p = a.p;
+    v.js = a.jstride;
+    v.ks = a.kstride;
+    v.ns = a.nstride;
+    v.b0 = a.begin[0];
+    v.b1 = a.begin[1];
+    v.b2 = a.begin[2];
+    return v;
+}
+
+bool holds(const mf_array4 &a, const int lo[3], const int hi[3], int nc) {
+    if (!a.p || a.ncomp < nc) return false;
+    for (int d = 0; d < 3; ++d)
+        if (hi[d] < lo[d] || lo[d] < a.begin[d] || hi[d] >= a.end[d]) return false;
+    return true;
+}
+
+bool valid_box(const mf_box &b) {
+    for (int d = 0; d < 3; ++d)
+        if (b.hi[d] < b.lo[d]) return false;
+    return true;
+}
+
+int grid_for(int64_t n) { return (int)std::min<int64_t>((n + kBlock - 1) / kBlock, kMaxGrid); }
+
+__global__ __launch_bounds__(kBlock) void k_average_down(Ext cb, int nv, View fine, View crse, int r, double volfrac) {
+    const int64_t tot = cb.cells * nv;
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < tot; q += (int64_t)gridDim.x * kBlock) {
+        int i, j, k, n;
+        cb.at(q, i, j, k, n);
+        double s = 0.0;
+        for (int kr = 0; kr < r; ++kr)
+            for (int jr = 0; jr < r; ++jr)
+                for (int ir = 0; ir < r; ++ir) s += fine(i * r + ir, j * r + jr, k * r + kr, n);
+        crse(i, j, k, n) = s * volfrac;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_time_interp(Ext b, int nv, View dst, View so, View sn, int mode, double alpha,
+                                                        double beta) {
+    const int64_t tot = b.cells * nv;
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < tot; q += (int64_t)gridDim.x * kBlock) {
+        int i, j, k, n;
+        b.at(q, i, j, k, n);
+        double v;
+        if (mode == 1) v = sn(i, j, k, n);
+        else if (mode == 0) v = so(i, j, k, n);
+        else v = rn(alpha * so(i, j, k, n)) + rn(beta * sn(i, j, k, n));
+        dst(i, j, k, n) = v;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_interp_cons(Ext fb, int nv, View c, View f, int r, double dfac) {
+    const int64_t tot = fb.cells * nv;
+    const double rr = (double)r;
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < tot; q += (int64_t)gridDim.x * kBlock) {
+        int i, j, k, n;
+        fb.at(q, i, j, k, n);
+        const int ic = floor_div(i, r), jc = floor_div(j, r), kc = floor_div(k, r);
+        const double u = c(ic, jc, kc, n);
+        double sx = mc_slope(c(ic - 1, jc, kc, n), u, c(ic + 1, jc, kc, n));
+        double sy = mc_slope(c(ic, jc - 1, kc, n), u, c(ic, jc + 1, kc, n));
+        double sz = mc_slope(c(ic, jc, kc - 1, n), u, c(ic, jc, kc + 1, n));
+        double alpha = 1.0;
+        if (sx != 0.0 || sy != 0.0 || sz != 0.0) {
+            const double dumax = (rn(fabs(sx) * dfac) + rn(fabs(sy) * dfac)) + rn(fabs(sz) * dfac);
+            double umax = u, umin = u;
+            for (int ko = -1; ko <= 1; ++ko)
+                for (int jo = -1; jo <= 1; ++jo)
+                    for (int io = -1; io <= 1; ++io) {
+                        const double v = c(ic + io, jc + jo, kc + ko, n);
+                        umax = fmax(umax, v);
+                        umin = fmin(umin, v);
+                    }
+            if (rn(dumax * alpha) > (umax - u)) alpha = (umax - u) / dumax;
+            if (rn(dumax * alpha) > (u - umin)) alpha = (u - umin) / dumax;
+        }
+        sx *= alpha;
+        sy *= alpha;
+        sz *= alpha;
+        const double xo = ((double)(i - ic * r) + 0.5) / rr - 0.5;
+        const double yo = ((double)(j - jc * r) + 0.5) / rr - 0.5;
+        const double zo = ((double)(k - kc * r) + 0.5) / rr - 0.5;
+        f(i, j, k, n) = ((u + rn(xo * sx)) + rn(yo * sy)) + rn(zo * sz);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_crse_add(Ext fb, int nv, View reg, View fl, double scale) {
+    const int64_t tot = fb.cells * nv;
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < tot; q += (int64_t)gridDim.x * kBlock) {
+        int i, j, k, n;
+        fb.at(q, i, j, k, n);
+        reg(i, j, k, n) = reg(i, j, k, n) + rn(scale * fl(i, j, k, n));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fine_add(Ext fb, int nv, View reg, View fl, int dir, int r, double scale) {
+    const int64_t tot = fb.cells * nv;
+    const int t1 = (dir == 0) ? 1 : 0, t2 = (dir == 2) ? 1 : 2;
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < tot; q += (int64_t)gridDim.x * kBlock) {
+        int i, j, k, n;
+        fb.at(q, i, j, k, n);
+        const int c[3] = {i, j, k};
+        double acc = reg(i, j, k, n);
+        for (int b = 0; b < r; ++b)
+            for (int a = 0; a < r; ++a) {
+                int p[3];
+                p[dir] = c[dir] * r;
+                p[t1] = c[t1] * r + a;
+                p[t2] = c[t2] * r + b;
+                acc = acc + rn(scale * fl(p[0], p[1], p[2], n));
+            }
+        reg(i, j, k, n) = acc;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_reflux(Ext fb, int nv, View s, View reg, int dir, int side, double mult,
+                                                   double vol) {
+    const int64_t tot = fb.cells * nv;
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < tot; q += (int64_t)gridDim.x * kBlock) {
+        int i, j, k, n;
+        fb.at(q, i, j, k, n);
+        const int sh = side == 0 ? 1 : 0;
+        const int ci = i - (dir == 0 ? sh : 0), cj = j - (dir == 1 ? sh : 0), ck = k - (dir == 2 ? sh : 0);
+        s(ci, cj, ck, n) = s(ci, cj, ck, n) + (mult * reg(i, j, k, n)) / vol;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mf_box_average_down(void *ctx, mf_box cbx, mf_array4 fine, mf_array4 crse, int ratio) {
+    hipStream_t st;
+    int nv;
+    if (!ctx || ratio < 1 || ratio > 8 || !valid_box(cbx)) return MF_ERR_BAD_ARG;
+    int rc = mf_ctx_stream(ctx, &st, &nv);
+    if (rc) return rc;
+    int flo[3], fhi[3];
+    for (int d = 0; d < 3; ++d) { flo[d] = cbx.lo[d] * ratio; fhi[d] = cbx.hi[d] * ratio + ratio - 1; }
+    if (!holds(crse, cbx.lo, cbx.hi, nv) || !holds(fine, flo, fhi, nv)) return MF_ERR_BAD_ARG;
+    const Ext e = ext_of(cbx);
+    hipLaunchKernelGGL(k_average_down, dim3(grid_for(e.cells * nv)), dim3(kBlock), 0, st, e, nv, view_of(fine),
+                       view_of(crse), ratio, 1.0 / (double)(ratio * ratio * ratio));
+    return hipGetLastError() == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+int mf_box_time_interp(void *ctx, mf_box bx, mf_array4 dst, mf_array4 s_old, mf_array4 s_new, double t_old,
+                       double t_new, double time) {
+    hipStream_t st;
+    int nv;
+    if (!ctx || !valid_box(bx)) return MF_ERR_BAD_ARG;
+    int rc = mf_ctx_stream(ctx, &st, &nv);
+    if (rc) return rc;
+    if (!holds(dst, bx.lo, bx.hi, nv) || !holds(s_old, bx.lo, bx.hi, nv) || !holds(s_new, bx.lo, bx.hi, nv))
+        return MF_ERR_BAD_ARG;
+    const double teps = (t_new - t_old) * 1.e-3;
+    int mode = 2;
+    if (time > t_new - teps && time < t_new + teps) mode = 1;
+    else if (time > t_old - teps && time < t_old + teps) mode = 0;
+    const double alpha = (mode == 2) ? (t_new - time) / (t_new - t_old) : 0.0;
+    const double beta = (mode == 2) ? (time - t_old) / (t_new - t_old) : 0.0;
+    const Ext e = ext_of(bx);
+    hipLaunchKernelGGL(k_time_interp, dim3(grid_for(e.cells * nv)), dim3(kBlock), 0, st, e, nv, view_of(dst),
+                       view_of(s_old), view_of(s_new), mode, alpha, beta);
+    return hipGetLastError() == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+int mf_box_interp_cons(void *ctx, mf_box fbx, mf_array4 crse, mf_array4 fine, int ratio) {
+    hipStream_t st;
+    int nv;
+    if (!ctx || ratio < 1 || ratio > 8 || !valid_box(fbx)) return MF_ERR_BAD_ARG;
+    int rc = mf_ctx_stream(ctx, &st, &nv);
+    if (rc) return rc;
+    int clo[3], chi[3];
+    for (int d = 0; d < 3; ++d) {
+        const int l = fbx.lo[d], h = fbx.hi[d];
+        clo[d] = ((l >= 0) ? l / ratio : -((-l + ratio - 1) / ratio)) - 1;
+        chi[d] = ((h >= 0) ? h / ratio : -((-h + ratio - 1) / ratio)) + 1;
+    }
+    if (!holds(fine, fbx.lo, fbx.hi, nv) || !holds(crse, clo, chi, nv)) return MF_ERR_BAD_ARG;
+    const Ext e = ext_of(fbx);
+    hipLaunchKernelGGL(k_interp_cons, dim3(grid_for(e.cells * nv)), dim3(kBlock), 0, st, e, nv, view_of(crse),
+                       view_of(fine), ratio, (double)(ratio - 1) / (double)(2 * ratio));
+    return hipGetLastError() == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+int mf_fluxreg_crse_add(void *ctx, int dir, mf_box face, mf_array4 reg, mf_array4 cflux, double scale) {
+    hipStream_t st;
+    int nv;
+    if (!ctx || dir < 0 || dir > 2 || !valid_box(face) || face.lo[dir] != face.hi[dir]) return MF_ERR_BAD_ARG;
+    int rc = mf_ctx_stream(ctx, &st, &nv);
+    if (rc) return rc;
+    if (!holds(reg, face.lo, face.hi, nv) || !holds(cflux, face.lo, face.hi, nv)) return MF_ERR_BAD_ARG;
+    const Ext e = ext_of(face);
+    hipLaunchKernelGGL(k_crse_add, dim3(grid_for(e.cells * nv)), dim3(kBlock), 0, st, e, nv, view_of(reg),
+                       view_of(cflux), scale);
+    return hipGetLastError() == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+int mf_fluxreg_fine_add(void *ctx, int dir, mf_box face, mf_array4 reg, mf_array4 fflux, int ratio, double scale) {
+    hipStream_t st;
+    int nv;
+    if (!ctx || dir < 0 || dir > 2 || ratio < 1 || ratio > 8 || !valid_box(face) || face.lo[dir] != face.hi[dir])
+        return MF_ERR_BAD_ARG;
+    int rc = mf_ctx_stream(ctx, &st, &nv);
+    if (rc) return rc;
+    const int t1 = (dir == 0) ? 1 : 0, t2 = (dir == 2) ? 1 : 2;
+    int flo[3], fhi[3];
+    flo[dir] = fhi[dir] = face.lo[dir] * ratio;
+    flo[t1] = face.lo[t1] * ratio;
+    fhi[t1] = face.hi[t1] * ratio + ratio - 1;
+    flo[t2] = face.lo[t2] * ratio;
+    fhi[t2] = face.hi[t2] * ratio + ratio - 1;
+    if (!holds(reg, face.lo, face.hi, nv) || !holds(fflux, flo, fhi, nv)) return MF_ERR_BAD_ARG;
+    const Ext e = ext_of(face);
+    hipLaunchKernelGGL(k_fine_add, dim3(grid_for(e.cells * nv)), dim3(kBlock), 0, st, e, nv, view_of(reg),
+                       view_of(fflux), dir, ratio, scale);
+    return hipGetLastError() == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+int mf_fluxreg_reflux(void *ctx, int dir, int side, mf_box face, mf_array4 state, mf_array4 reg, double scale,
+                      double vol) {
+    hipStream_t st;
+    int nv;
+    if (!ctx || dir < 0 || dir > 2 || (side != 0 && side != 1) || !(vol > 0.0) || !valid_box(face) ||
+        face.lo[dir] != face.hi[dir])
+        return MF_ERR_BAD_ARG;
+    int rc = mf_ctx_stream(ctx, &st, &nv);
+    if (rc) return rc;
+    mf_box cells = face;
+    if (side == 0) { cells.lo[dir] -= 1; cells.hi[dir] -= 1; }
+    if (!holds(reg, face.lo, face.hi, nv) || !holds(state, cells.lo, cells.hi, nv)) return MF_ERR_BAD_ARG;
+    const Ext e = ext_of(face);
+    hipLaunchKernelGGL(k_reflux, dim3(grid_for(e.cells * nv)), dim3(kBlock), 0, st, e, nv, view_of(state),
+                       view_of(reg), dir, side, side == 0 ? -scale : scale, vol);
+    return hipGetLastError() == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+}  // extern "C"

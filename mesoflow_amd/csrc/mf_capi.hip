@@ -1992,3 +1992,12 @@ int mf_level_dt_prev(void *level, double *dt_prev) {
     *dt_prev = h.dt_prev;
     return MF_OK;
 }
+
+// stream and component count of a context, for mf_amr.hip
+int mf_ctx_stream(void *ctx, hipStream_t *stream, int *nv) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c || !stream || !nv) return MF_ERR_BAD_ARG;
+    *stream = c->stream;
+    *nv = c->nv;
+    return MF_OK;
+}

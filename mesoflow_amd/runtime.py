@@ -32,6 +32,14 @@ SIGNATURES = {
     "mf_box_cutcells": ([C.c_void_p, abi.Box, abi.Array4, C.c_double], C.c_int),
     "mf_box_rhs": ([C.c_void_p, abi.Box, abi.Array4, abi.Array4, _P(abi.Array4), C.c_double], C.c_int),
     "mf_box_rk3_combine": ([C.c_void_p, C.c_int, abi.Box, abi.Array4, abi.Array4, abi.Array4, C.c_double], C.c_int),
+    "mf_box_average_down": ([C.c_void_p, abi.Box, abi.Array4, abi.Array4, C.c_int], C.c_int),
+    "mf_box_time_interp": ([C.c_void_p, abi.Box, abi.Array4, abi.Array4, abi.Array4, C.c_double, C.c_double,
+                            C.c_double], C.c_int),
+    "mf_box_interp_cons": ([C.c_void_p, abi.Box, abi.Array4, abi.Array4, C.c_int], C.c_int),
+    "mf_fluxreg_crse_add": ([C.c_void_p, C.c_int, abi.Box, abi.Array4, abi.Array4, C.c_double], C.c_int),
+    "mf_fluxreg_fine_add": ([C.c_void_p, C.c_int, abi.Box, abi.Array4, abi.Array4, C.c_int, C.c_double], C.c_int),
+    "mf_fluxreg_reflux": ([C.c_void_p, C.c_int, C.c_int, abi.Box, abi.Array4, abi.Array4, C.c_double, C.c_double],
+                          C.c_int),
     "mf_level_create": ([C.c_void_p, _P(abi.Box), C.c_int, _P(C.c_void_p)], C.c_int),
     "mf_level_destroy": ([C.c_void_p], C.c_int),
     "mf_level_set_state": ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),

@@ -59,6 +59,15 @@ def lib():
         L.orc_box_cutcells.restype = C.c_int
         L.orc_kat.argtypes = [P(abi.PluginParams), C.c_int, C.c_void_p, C.c_void_p]
         L.orc_kat.restype = C.c_int
+        A4, B = abi.Array4, abi.Box
+        for name, args in (("orc_average_down", [B, A4, A4, C.c_int, C.c_int]),
+                           ("orc_time_interp", [B, A4, A4, A4, C.c_double, C.c_double, C.c_double, C.c_int]),
+                           ("orc_interp_cons", [B, A4, A4, C.c_int, C.c_int]),
+                           ("orc_fluxreg_crse_add", [C.c_int, B, A4, A4, C.c_double, C.c_int]),
+                           ("orc_fluxreg_fine_add", [C.c_int, B, A4, A4, C.c_int, C.c_double, C.c_int]),
+                           ("orc_fluxreg_reflux", [C.c_int, C.c_int, B, A4, A4, C.c_double, C.c_double, C.c_int])):
+            getattr(L, name).argtypes = args
+            getattr(L, name).restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -188,3 +197,47 @@ def box_cutcells(case, lo, hi, sborder):
     if st != 0:
         raise abi.MFError("orc_box_cutcells", st)
     return sb
+
+
+# --------------------------------------------------------------------------
+# two-level coupling operators (mf_amr_oracle.cpp) on numpy fabs: a fab is
+# (array [nc][nz][ny][nx], lo (x, y, z) of its first cell or face)
+# --------------------------------------------------------------------------
+def _a4(fab):
+    a, lo = fab
+    assert a.flags.c_contiguous and a.dtype == np.float64
+    nc, nz, ny, nx = a.shape
+    return abi.make_array4(a.ctypes.data, lo, (lo[0] + nx, lo[1] + ny, lo[2] + nz), nc)
+
+
+def _ok(st, where):
+    if st != 0:
+        raise abi.MFError(where, st)
+
+
+def average_down(cbx, fine, crse, ratio):
+    _ok(lib().orc_average_down(abi.make_box(*cbx), _a4(fine), _a4(crse), ratio, crse[0].shape[0]), "orc_average_down")
+
+
+def time_interp(bx, dst, s_old, s_new, t_old, t_new, time):
+    _ok(lib().orc_time_interp(abi.make_box(*bx), _a4(dst), _a4(s_old), _a4(s_new), t_old, t_new, time,
+                              dst[0].shape[0]), "orc_time_interp")
+
+
+def interp_cons(fbx, crse, fine, ratio):
+    _ok(lib().orc_interp_cons(abi.make_box(*fbx), _a4(crse), _a4(fine), ratio, fine[0].shape[0]), "orc_interp_cons")
+
+
+def fluxreg_crse_add(d, face, reg, cflux, scale):
+    _ok(lib().orc_fluxreg_crse_add(d, abi.make_box(*face), _a4(reg), _a4(cflux), scale, reg[0].shape[0]),
+        "orc_fluxreg_crse_add")
+
+
+def fluxreg_fine_add(d, face, reg, fflux, ratio, scale):
+    _ok(lib().orc_fluxreg_fine_add(d, abi.make_box(*face), _a4(reg), _a4(fflux), ratio, scale, reg[0].shape[0]),
+        "orc_fluxreg_fine_add")
+
+
+def fluxreg_reflux(d, side, face, state, reg, scale, vol):
+    _ok(lib().orc_fluxreg_reflux(d, side, abi.make_box(*face), _a4(state), _a4(reg), scale, vol, reg[0].shape[0]),
+        "orc_fluxreg_reflux")

@@ -1,0 +1,202 @@
+"""Pins of the oracle's two-level coupling operators (oracle/mf_amr_oracle.cpp):
+average_down, FillPatch's time interpolation, cell_cons_interp and the
+FluxRegister operations, by closed forms and invariants: exact sums on integer
+data, exactness on linear fields, conservation and boundedness of the
+interpolation, restriction of the interpolation = identity, and conservation
+of a refluxed two-level update with periodic boundaries. AMReX itself is not
+available, so agreement with its implementation is unpinned."""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+
+def _fab(lo, hi, nc, fill=0.0):
+    n = [hi[d] - lo[d] + 1 for d in range(3)]
+    return (np.full((nc, n[2], n[1], n[0]), fill), tuple(lo))
+
+
+def _at(fab, i, j, k, n):
+    a, lo = fab
+    return a[n, k - lo[2], j - lo[1], i - lo[0]]
+
+
+def _centres(lo, hi, h):
+    """cell-centre coordinates of the box cells, arrays shaped [nz][ny][nx]"""
+    ax = [(np.arange(lo[d], hi[d] + 1) + 0.5) * h for d in range(3)]
+    z, y, x = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    return x, y, z
+
+
+@pytest.mark.parametrize("r", [2, 3])
+def test_average_down_integer_sums(r):
+    rng = np.random.default_rng(r)
+    cbx = ((1, -2, 3), (4, 1, 5))
+    flo = tuple(cbx[0][d] * r - 1 for d in range(3))
+    fhi = tuple(cbx[1][d] * r + r for d in range(3))
+    fine = _fab(flo, fhi, 2)
+    fine[0][...] = rng.integers(-1000, 1000, size=fine[0].shape)
+    crse = _fab((0, -3, 2), (5, 2, 6), 2, np.nan)
+    po.average_down(cbx, fine, crse, r)
+    for k, j, i in itertools.product(range(cbx[0][2], cbx[1][2] + 1), range(cbx[0][1], cbx[1][1] + 1),
+                                     range(cbx[0][0], cbx[1][0] + 1)):
+        for n in range(2):
+            s = sum(_at(fine, i * r + a, j * r + b, k * r + c, n) for a in range(r) for b in range(r) for c in range(r))
+            assert _at(crse, i, j, k, n) == s * (1.0 / r ** 3)  # the integer sum is exact
+    assert np.isnan(_at(crse, 0, -3, 2, 0))  # outside cbx: untouched
+
+
+def test_average_down_linear_field():
+    r = 2
+    cbx = ((0, 0, 0), (3, 2, 4))
+    fine = _fab((0, 0, 0), (7, 5, 9), 1)
+    x, y, z = _centres((0, 0, 0), (7, 5, 9), 0.5)
+    fine[0][0] = 3.0 + 0.5 * x - 0.25 * y + 2.0 * z
+    crse = _fab((0, 0, 0), (3, 2, 4), 1)
+    po.average_down(cbx, fine, crse, r)
+    x, y, z = _centres((0, 0, 0), (3, 2, 4), 1.0)
+    assert np.array_equal(crse[0][0], 3.0 + 0.5 * x - 0.25 * y + 2.0 * z)
+
+
+def test_time_interp_selection_and_linearity():
+    bx = ((0, 0, 0), (3, 2, 1))
+    rng = np.random.default_rng(1)
+    so = _fab(*bx, 3)
+    sn = _fab(*bx, 3)
+    so[0][...] = rng.standard_normal(so[0].shape)
+    sn[0][...] = rng.standard_normal(sn[0].shape)
+    t0, t1 = 0.25, 0.75
+    for time, want in ((t0, so[0]), (t1, sn[0]), (t1 - 1e-4 * (t1 - t0), sn[0]), (t0 + 1e-4 * (t1 - t0), so[0])):
+        d = _fab(*bx, 3, np.nan)
+        po.time_interp(bx, d, so, sn, t0, t1, time)
+        assert np.array_equal(d[0], want)
+    for time in (0.3, 0.5, 0.7, 1.0):
+        d = _fab(*bx, 3, np.nan)
+        po.time_interp(bx, d, so, sn, t0, t1, time)
+        beta = (time - t0) / (t1 - t0)
+        assert np.allclose(d[0], so[0] + beta * (sn[0] - so[0]), rtol=0, atol=4e-16 * np.abs(so[0]).max() * 4)
+    c = _fab(*bx, 1, 7.25)
+    d = _fab(*bx, 1)
+    po.time_interp(bx, d, c, c, t0, t1, 0.5)
+    assert np.all(d[0] == 7.25)
+
+
+def _interp(cr, fbx, r, nc=1):
+    fine = _fab(*fbx, nc, np.nan)
+    po.interp_cons(fbx, cr, fine, r)
+    return fine
+
+
+@pytest.mark.parametrize("r", [2, 3])
+def test_interp_constant_and_linear(r):
+    clo, chi = (-2, -1, 0), (5, 4, 6)
+    fbx = ((-1 * r, 0, 1 * r), (4 * r + r - 1, 3 * r + r - 1, 5 * r + r - 1))
+    cr = _fab(clo, chi, 1, -3.5)
+    assert np.all(_interp(cr, fbx, r)[0] == -3.5)
+    x, y, z = _centres(clo, chi, 1.0)
+    cr[0][0] = 1.0 + 0.75 * x - 1.5 * y + 0.125 * z
+    f = _interp(cr, fbx, r)
+    x, y, z = _centres(*fbx, 1.0 / r)
+    want = 1.0 + 0.75 * x - 1.5 * y + 0.125 * z
+    assert np.max(np.abs(f[0][0] - want)) <= 1e-14 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("r", [2, 3])
+def test_interp_conservative_bounded_and_restricts(r):
+    rng = np.random.default_rng(7 + r)
+    clo, chi = (0, 0, 0), (7, 6, 5)
+    cr = _fab(clo, chi, 2)
+    cr[0][...] = rng.standard_normal(cr[0].shape) * np.array([1.0, 1e5])[:, None, None, None]
+    cbx = ((1, 1, 1), (6, 5, 4))
+    fbx = (tuple(cbx[0][d] * r for d in range(3)), tuple(cbx[1][d] * r + r - 1 for d in range(3)))
+    f = _interp(cr, fbx, r, 2)
+    back = _fab(*cbx, 2, np.nan)
+    po.average_down(cbx, f, back, r)
+    for n in range(2):
+        sub = cr[0][n, 1:5, 1:6, 1:7]
+        assert np.max(np.abs(back[0][n] - sub)) <= 8e-16 * np.abs(cr[0][n]).max()
+    limited = 0
+    for k, j, i in itertools.product(range(cbx[0][2], cbx[1][2] + 1), range(cbx[0][1], cbx[1][1] + 1),
+                                     range(cbx[0][0], cbx[1][0] + 1)):
+        for n in range(2):
+            nb = cr[0][n, k - 1:k + 2, j - 1:j + 2, i - 1:i + 2]
+            kids = f[0][n, (k - cbx[0][2]) * r:(k - cbx[0][2] + 1) * r, (j - cbx[0][1]) * r:(j - cbx[0][1] + 1) * r,
+                        (i - cbx[0][0]) * r:(i - cbx[0][0] + 1) * r]
+            tol = 1e-15 * np.abs(nb).max()
+            assert kids.min() >= nb.min() - tol and kids.max() <= nb.max() + tol
+            limited += int(kids.max() >= nb.max() - tol or kids.min() <= nb.min() + tol)
+    assert limited > 0   # the bound is reached somewhere: the limiter acted
+
+
+def test_interp_local_extremum_is_flat():
+    cr = _fab((0, 0, 0), (2, 2, 2), 1, 1.0)
+    cr[0][0, 1, 1, 1] = 5.0
+    cr[0][0, 0, 1, 1] = 2.0   # the slopes in x, y and z all change sign at the centre
+    f = _interp(cr, ((2, 2, 2), (3, 3, 3)), 2)
+    assert np.all(f[0] == 5.0)
+
+
+def test_reflux_conserves_a_two_level_update():
+    """periodic coarse domain 8^3 (dx 1), fine region = coarse cells 2..5 (ratio
+    2): coarse and fine cells updated with their own random face fluxes, the
+    coarse-fine mismatch refluxed and the fine level averaged down; the
+    composite total is unchanged and only cells next to the fine region move"""
+    rng = np.random.default_rng(3)
+    N, r, nc, dt = 8, 2, 2, 0.1
+    Uc = rng.standard_normal((nc, N, N, N))
+    cl, ch = 2, 5
+    nf = (ch - cl + 1) * r
+    Uf = rng.standard_normal((nc, nf, nf, nf))
+    M0 = Uc.sum(axis=(1, 2, 3)) - Uc[:, cl:ch + 1, cl:ch + 1, cl:ch + 1].sum(axis=(1, 2, 3)) + Uf.sum(axis=(1, 2, 3)) / 8
+    Fc = [rng.standard_normal((nc, N, N, N)) for _ in range(3)]    # Fc[d][.., i] = flux through the low face of cell i
+    Ff = []
+    for d in range(3):
+        shp = [nf, nf, nf]
+        shp[2 - d] += 1
+        Ff.append(rng.standard_normal([nc] + shp))
+    Uc2 = Uc.copy()
+    for d in range(3):
+        ax = 3 - d
+        Uc2 -= dt / 1.0 * (np.roll(Fc[d], -1, axis=ax) - Fc[d])
+    Uf2 = Uf.copy()
+    for d in range(3):
+        ax = 3 - d
+        hi = [slice(None)] * 4
+        lo = [slice(None)] * 4
+        hi[ax] = slice(1, None)
+        lo[ax] = slice(0, -1)
+        Uf2 -= dt / 0.5 * (Ff[d][tuple(hi)] - Ff[d][tuple(lo)])
+    state = (Uc2.copy(), (0, 0, 0))
+    before = state[0].copy()
+    for d in range(3):
+        for side in (0, 1):
+            fidx = cl if side == 0 else ch + 1
+            face = [[cl, ch], [cl, ch], [cl, ch]]
+            face[d] = [fidx, fidx]
+            flo = tuple(face[q][0] for q in range(3))
+            fhi = tuple(face[q][1] for q in range(3))
+            reg = _fab(flo, fhi, nc)
+            cf = _fab(flo, fhi, nc)
+            sl = [slice(None)] + [slice(face[q][0] % N, face[q][0] % N + face[q][1] - face[q][0] + 1) for q in (2, 1, 0)]
+            cf[0][...] = Fc[d][tuple(sl)]
+            po.fluxreg_crse_add(d, (flo, fhi), reg, cf, -dt * 1.0)
+            ff = (np.ascontiguousarray(Ff[d]), tuple(cl * r for _ in range(3)))
+            po.fluxreg_fine_add(d, (flo, fhi), reg, ff, r, dt * 0.25)
+            po.fluxreg_reflux(d, side, (flo, fhi), state, reg, 1.0, 1.0)
+    moved = np.any(state[0] != before, axis=0)
+    touch = np.zeros((N, N, N), bool)   # cells outside the fine region that share a face with it
+    for ax in range(3):
+        for c in (cl - 1, ch + 1):
+            sl = [slice(cl, ch + 1)] * 3
+            sl[ax] = c
+            touch[tuple(sl)] = True
+    assert np.array_equal(moved, touch)
+    po.average_down(((cl,) * 3, (ch,) * 3), (np.ascontiguousarray(Uf2), (cl * r,) * 3), state, r)
+    M1 = state[0].sum(axis=(1, 2, 3))
+    assert np.allclose(M1, M0, rtol=0, atol=1e-12 * np.abs(Uc).sum())
+    # the same without the reflux is not conservative
+    bad = (Uc2.copy(), (0, 0, 0))
+    po.average_down(((cl,) * 3, (ch,) * 3), (np.ascontiguousarray(Uf2), (cl * r,) * 3), bad, r)
+    assert np.max(np.abs(bad[0].sum(axis=(1, 2, 3)) - M0)) > 1e-3
