@@ -229,6 +229,16 @@ int mf_fluxreg_fine_add(void *ctx, int dir, mf_box face, mf_array4 reg, mf_array
 int mf_fluxreg_reflux(void *ctx, int dir, int side, mf_box face, mf_array4 state, mf_array4 reg, double scale,
                       double vol);
 
+/* ErrorEst (Source/mflo.cpp:183-260) on one box: state holds bx grown by one
+ * (ghost-filled); tags is device memory, one byte per cell of bx (x fastest).
+ * For each of the ntag (<= 16) variables, state_based_refinement and then
+ * stategrad_based_refinement (Source/mflo_tagging.H:6-70) set the byte to
+ * tagval where |state(comps[c])| > refine_phi[c] or the largest one-sided
+ * absolute difference in x, y, z exceeds refine_phigrad[c]. Tags are only
+ * set, never cleared. The parameter arrays are host memory. */
+int mf_box_tag(void *ctx, mf_box bx, mf_array4 state, unsigned char *tags, const double *refine_phi,
+               const double *refine_phigrad, const int *comps, int ntag, int tagval);
+
 /* ------------------------------------------------------------------------ */
 /* Level forms: one AMR level of boxes stored as one domain-sized array with  */
 /* MF_NGROW ghost layers. Boxes must tile the domain (level 0).               */
@@ -298,6 +308,12 @@ int mf_level_evolve_split(void *level, double t_ss, double t_react, double dt_re
 int mf_level_info(void *level, double *time, double *dt, int *istep, double *resnorms);
 /* Overrides the level time/dt history (restart). */
 int mf_level_set_time(void *level, double time, double dt_prev, int istep);
+
+/* ErrorEst over a whole level: ghost cells filled at time (FillPatch), tags
+ * (host memory, one byte per domain cell, x fastest) cleared and set to 1
+ * where a criterion of mf_box_tag holds; *ntagged (may be NULL) counts them. */
+int mf_level_tag(void *level, double time, const double *refine_phi, const double *refine_phigrad, const int *comps,
+                 int ntag, unsigned char *tags, int64_t *ntagged);
 
 /* Box list, geometry and component count of a level (boxes may be NULL to
  * query the count; cap is the capacity of boxes). */

@@ -1,9 +1,14 @@
 """Two-level coupling operators on device fabs (mf_box_average_down,
-mf_box_time_interp, mf_box_interp_cons, mf_fluxreg_*): the AMReX calls of
+mf_box_time_interp, mf_box_interp_cons, mf_fluxreg_*) and the tagging of
+ErrorEst (mf_box_tag, mf_level_tag): the AMReX calls of
 Source/mflo_gridwork.cpp:125-193, Source/mflo.cpp:310-330 and
 Source/mflo_evolve.cpp:232-248, 655-681. A fab is (tensor [nc][nz][ny][nx]
 on the device, lo (x, y, z) of its first cell or face). Work is enqueued on
 the context's stream; Context.sync() waits for it."""
+import ctypes as C
+
+import numpy as np
+
 from . import abi, runtime
 
 
@@ -42,3 +47,28 @@ def fluxreg_fine_add(ctx, d, face, reg, fflux, ratio, scale):
 def fluxreg_reflux(ctx, d, side, face, state, reg, scale, vol):
     runtime.check(runtime.lib().mf_fluxreg_reflux(ctx.h, d, side, abi.make_box(*face), _a4(state), _a4(reg),
                                                   float(scale), float(vol)), "mf_fluxreg_reflux")
+
+
+def _tag_params(refine_phi, refine_phigrad, comps):
+    n = len(comps)
+    return ((C.c_double * max(n, 1))(*refine_phi), (C.c_double * max(n, 1))(*refine_phigrad),
+            (C.c_int * max(n, 1))(*comps), n)
+
+
+def box_tag(ctx, bx, state, tags, refine_phi, refine_phigrad, comps, tagval=1):
+    """ErrorEst on one box (Source/mflo.cpp:183-260); tags: uint8 device tensor [nz][ny][nx] over bx"""
+    ph, gr, cp, n = _tag_params(refine_phi, refine_phigrad, comps)
+    assert tags.is_contiguous() and tags.is_cuda
+    runtime.check(runtime.lib().mf_box_tag(ctx.h, abi.make_box(*bx), _a4(state), C.c_void_p(tags.data_ptr()), ph, gr,
+                                           cp, n, tagval), "mf_box_tag")
+
+
+def level_tag(level, time, refine_phi, refine_phigrad, comps):
+    """ErrorEst over a level: (tags uint8 [nz][ny][nx] on the host, number tagged)"""
+    ph, gr, cp, n = _tag_params(refine_phi, refine_phigrad, comps)
+    nx, ny, nz = level.ctx.case.n_cell
+    tags = np.zeros((nz, ny, nx), dtype=np.uint8)
+    cnt = C.c_int64()
+    runtime.check(runtime.lib().mf_level_tag(level.h, float(time), ph, gr, cp, n, tags.ctypes.data_as(C.c_void_p),
+                                             C.byref(cnt)), "mf_level_tag")
+    return tags, cnt.value

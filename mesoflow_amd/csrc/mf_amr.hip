@@ -9,6 +9,7 @@
 //                           (Source/mflo_gridwork.cpp:151-165, 170-193)
 //   FluxRegister            CrseInit / FineAdd (Source/mflo_evolve.cpp:655-681),
 //                           Reflux (Source/mflo_evolve.cpp:243-246)
+//   tagging                 ErrorEst (Source/mflo.cpp:183-260, Source/mflo_tagging.H)
 // One thread per output value (cell or face, component); every product that
 // feeds an add is rounded on its own (no FMA), so the results equal the CPU
 // restatement's bit for bit.
@@ -199,9 +200,104 @@ __global__ __launch_bounds__(kBlock) void k_reflux(Ext fb, int nv, View s, View 
     }
 }
 
+constexpr int kMaxTagVars = 16;
+struct TagVars {
+    int n;
+    int comp[kMaxTagVars];
+    double phi[kMaxTagVars], grad[kMaxTagVars];
+};
+
+// ErrorEst's two criteria per cell (Source/mflo_tagging.H:6-70); tags dense over bx
+__global__ __launch_bounds__(kBlock) void k_tag(Ext b, View s, unsigned char *tags, TagVars tv, unsigned char tagval) {
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < b.cells; q += (int64_t)gridDim.x * kBlock) {
+        int i, j, k, n0;
+        b.at(q, i, j, k, n0);
+        bool set = false;
+        for (int c = 0; c < tv.n; ++c)
+            if (fabs(s(i, j, k, tv.comp[c])) > tv.phi[c]) set = true;
+        for (int c = 0; c < tv.n; ++c) {
+            const int n = tv.comp[c];
+            const double u = s(i, j, k, n);
+            const double gx = fmax(fabs(s(i + 1, j, k, n) - u), fabs(u - s(i - 1, j, k, n)));
+            const double gy = fmax(fabs(s(i, j + 1, k, n) - u), fabs(u - s(i, j - 1, k, n)));
+            const double gz = fmax(fabs(s(i, j, k + 1, n) - u), fabs(u - s(i, j, k - 1, n)));
+            if (fmax(gx, fmax(gy, gz)) > tv.grad[c]) set = true;
+        }
+        if (set) tags[q] = tagval;
+    }
+}
+
+int tag_vars(const double *phi, const double *grad, const int *comps, int ntag, int nv, TagVars &tv) {
+    if (ntag < 0 || ntag > kMaxTagVars || (ntag > 0 && (!phi || !grad || !comps))) return MF_ERR_BAD_ARG;
+    tv.n = ntag;
+    for (int c = 0; c < ntag; ++c) {
+        if (comps[c] < 0 || comps[c] >= nv) return MF_ERR_BAD_ARG;
+        tv.comp[c] = comps[c];
+        tv.phi[c] = phi[c];
+        tv.grad[c] = grad[c];
+    }
+    return MF_OK;
+}
+
 }  // namespace
 
+int mf_level_ctx(void *level, void **ctx);  // mf_capi.hip
+
 extern "C" {
+
+int mf_box_tag(void *ctx, mf_box bx, mf_array4 state, unsigned char *tags, const double *refine_phi,
+               const double *refine_phigrad, const int *comps, int ntag, int tagval) {
+    hipStream_t st;
+    int nv;
+    if (!ctx || !tags || !valid_box(bx)) return MF_ERR_BAD_ARG;
+    int rc = mf_ctx_stream(ctx, &st, &nv);
+    if (rc) return rc;
+    TagVars tv;
+    rc = tag_vars(refine_phi, refine_phigrad, comps, ntag, nv, tv);
+    if (rc) return rc;
+    int glo[3], ghi[3];
+    for (int d = 0; d < 3; ++d) { glo[d] = bx.lo[d] - 1; ghi[d] = bx.hi[d] + 1; }
+    if (!holds(state, glo, ghi, nv)) return MF_ERR_BAD_ARG;
+    if (ntag == 0) return MF_OK;
+    const Ext e = ext_of(bx);
+    hipLaunchKernelGGL(k_tag, dim3(grid_for(e.cells)), dim3(kBlock), 0, st, e, view_of(state), tags, tv,
+                       (unsigned char)tagval);
+    return hipGetLastError() == hipSuccess ? MF_OK : MF_ERR_HIP;
+}
+
+int mf_level_tag(void *level, double time, const double *refine_phi, const double *refine_phigrad, const int *comps,
+                 int ntag, unsigned char *tags, int64_t *ntagged) {
+    if (!level || !tags) return MF_ERR_BAD_ARG;
+    mf_array4 v;
+    mf_geom g;
+    int rc = mf_level_state_view(level, &v);
+    if (!rc) rc = mf_level_boxes(level, nullptr, 0, nullptr, &g, nullptr);
+    if (!rc) rc = mf_fill_ghosts_single_level(level, time);
+    if (rc) return rc;
+    mf_box dom;
+    for (int d = 0; d < 3; ++d) { dom.lo[d] = 0; dom.hi[d] = g.n_cell[d] - 1; }
+    const int64_t ncell = (int64_t)g.n_cell[0] * g.n_cell[1] * g.n_cell[2];
+    hipStream_t st = nullptr;
+    int nv = 0;
+    void *ctx = nullptr;
+    rc = mf_level_ctx(level, &ctx);
+    if (!rc) rc = mf_ctx_stream(ctx, &st, &nv);
+    if (rc) return rc;
+    unsigned char *dt = nullptr;
+    if (hipMalloc(&dt, ncell) != hipSuccess) return MF_ERR_HIP;
+    if (hipMemsetAsync(dt, 0, ncell, st) != hipSuccess) rc = MF_ERR_HIP;
+    if (!rc) rc = mf_box_tag(ctx, dom, v, dt, refine_phi, refine_phigrad, comps, ntag, 1);
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = MF_ERR_HIP;
+    if (!rc && hipMemcpy(tags, dt, ncell, hipMemcpyDeviceToHost) != hipSuccess) rc = MF_ERR_HIP;
+    hipFree(dt);
+    if (rc) return rc;
+    if (ntagged) {
+        int64_t cnt = 0;
+        for (int64_t q = 0; q < ncell; ++q) cnt += tags[q] != 0;
+        *ntagged = cnt;
+    }
+    return MF_OK;
+}
 
 int mf_box_average_down(void *ctx, mf_box cbx, mf_array4 fine, mf_array4 crse, int ratio) {
     hipStream_t st;

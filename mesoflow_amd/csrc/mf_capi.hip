@@ -2001,3 +2001,11 @@ int mf_ctx_stream(void *ctx, hipStream_t *stream, int *nv) {
     *nv = c->nv;
     return MF_OK;
 }
+
+// context of a level, for mf_amr.hip
+int mf_level_ctx(void *level, void **ctx) {
+    Level *L = (Level *)level;
+    if (!L || !ctx) return MF_ERR_BAD_ARG;
+    *ctx = L->c;
+    return MF_OK;
+}

@@ -40,6 +40,10 @@ SIGNATURES = {
     "mf_fluxreg_fine_add": ([C.c_void_p, C.c_int, abi.Box, abi.Array4, abi.Array4, C.c_int, C.c_double], C.c_int),
     "mf_fluxreg_reflux": ([C.c_void_p, C.c_int, C.c_int, abi.Box, abi.Array4, abi.Array4, C.c_double, C.c_double],
                           C.c_int),
+    "mf_box_tag": ([C.c_void_p, abi.Box, abi.Array4, C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_int), C.c_int,
+                    C.c_int], C.c_int),
+    "mf_level_tag": ([C.c_void_p, C.c_double, _P(C.c_double), _P(C.c_double), _P(C.c_int), C.c_int, C.c_void_p,
+                      _P(C.c_int64)], C.c_int),
     "mf_level_create": ([C.c_void_p, _P(abi.Box), C.c_int, _P(C.c_void_p)], C.c_int),
     "mf_level_destroy": ([C.c_void_p], C.c_int),
     "mf_level_set_state": ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),

@@ -4,7 +4,8 @@
 // AMReX (Source/mflo_gridwork.cpp:125-193, Source/mflo.cpp:310-330,
 // Source/mflo_evolve.cpp:232-248, 655-681): average_down, the time
 // interpolation of FillPatch, the conservative linear interpolation
-// (cell_cons_interp) and the FluxRegister operations. AMReX is not in the
+// (cell_cons_interp) and the FluxRegister operations; and the tagging of
+// ErrorEst (Source/mflo.cpp:183-260, Source/mflo_tagging.H). AMReX is not in the
 // tree: the algorithms are restated from AMReX's published descriptions, so
 // parity with AMReX itself is unpinned; tests/test_oracle_amr.py pins these
 // functions with closed forms and invariants (exactness on linear data,
@@ -15,6 +16,7 @@
 // indices. A face box (FluxRegister) holds face indices: along dir its lo ==
 // hi is the face index, the other directions are cell ranges.
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include "mesoflow_amd.h"
@@ -238,6 +240,48 @@ int orc_fluxreg_reflux(int dir, int side, mf_box face, mf_array4 state, mf_array
                     const double t = (mult * r(i, j, k, n)) / vol;
                     s(ci, cj, ck, n) = s(ci, cj, ck, n) + t;
                 }
+    return MF_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// ErrorEst (Source/mflo.cpp:183-260) on one box: state holds bx grown by one
+// (FillPatch); tags is a dense byte array over bx (x fastest). For each tagged
+// variable c, state_based_refinement (Source/mflo_tagging.H:6-27) sets the
+// tag where |state(comp_c)| > refine_phi[c], then stategrad_based_refinement
+// (:29-70) where the largest of the one-sided absolute differences in x, y
+// and z exceeds refine_phigrad[c]. Tags are only ever set.
+int orc_box_tag(mf_box bx, mf_array4 state, unsigned char *tags, const double *refine_phi,
+                const double *refine_phigrad, const int *comps, int ntag, int tagval) {
+    if (!tags || ntag < 0 || (ntag > 0 && (!refine_phi || !refine_phigrad || !comps))) return MF_ERR_BAD_ARG;
+    int glo[3], ghi[3];
+    for (int d = 0; d < 3; ++d) { glo[d] = bx.lo[d] - 1; ghi[d] = bx.hi[d] + 1; }
+    int maxc = 0;
+    for (int c = 0; c < ntag; ++c) {
+        if (comps[c] < 0) return MF_ERR_BAD_ARG;
+        maxc = std::max(maxc, comps[c] + 1);
+    }
+    if (!holds(state, glo, ghi, maxc)) return MF_ERR_BAD_ARG;
+    const V s = mk(state);
+    const int ex = bx.hi[0] - bx.lo[0] + 1, ey = bx.hi[1] - bx.lo[1] + 1;
+    const unsigned char tv = (unsigned char)tagval;
+    for (int k = bx.lo[2]; k <= bx.hi[2]; ++k)
+        for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
+            for (int i = bx.lo[0]; i <= bx.hi[0]; ++i) {
+                unsigned char &t = tags[(int64_t)(i - bx.lo[0]) + (int64_t)ex * ((j - bx.lo[1]) + (int64_t)ey * (k - bx.lo[2]))];
+                for (int c = 0; c < ntag; ++c)
+                    if (std::fabs(s(i, j, k, comps[c])) > refine_phi[c]) t = tv;
+                for (int c = 0; c < ntag; ++c) {
+                    const int n = comps[c];
+                    const double u = s(i, j, k, n);
+                    const double gx = std::max(std::fabs(s(i + 1, j, k, n) - u), std::fabs(u - s(i - 1, j, k, n)));
+                    const double gy = std::max(std::fabs(s(i, j + 1, k, n) - u), std::fabs(u - s(i, j - 1, k, n)));
+                    const double gz = std::max(std::fabs(s(i, j, k + 1, n) - u), std::fabs(u - s(i, j, k - 1, n)));
+                    if (std::max(gx, std::max(gy, gz)) > refine_phigrad[c]) t = tv;
+                }
+            }
     return MF_OK;
 }
 

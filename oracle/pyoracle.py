@@ -65,7 +65,9 @@ def lib():
                            ("orc_interp_cons", [B, A4, A4, C.c_int, C.c_int]),
                            ("orc_fluxreg_crse_add", [C.c_int, B, A4, A4, C.c_double, C.c_int]),
                            ("orc_fluxreg_fine_add", [C.c_int, B, A4, A4, C.c_int, C.c_double, C.c_int]),
-                           ("orc_fluxreg_reflux", [C.c_int, C.c_int, B, A4, A4, C.c_double, C.c_double, C.c_int])):
+                           ("orc_fluxreg_reflux", [C.c_int, C.c_int, B, A4, A4, C.c_double, C.c_double, C.c_int]),
+                           ("orc_box_tag", [B, A4, C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int), C.c_int,
+                                            C.c_int])):
             getattr(L, name).argtypes = args
             getattr(L, name).restype = C.c_int
         _LIB = L
@@ -241,3 +243,13 @@ def fluxreg_fine_add(d, face, reg, fflux, ratio, scale):
 def fluxreg_reflux(d, side, face, state, reg, scale, vol):
     _ok(lib().orc_fluxreg_reflux(d, side, abi.make_box(*face), _a4(state), _a4(reg), scale, vol, reg[0].shape[0]),
         "orc_fluxreg_reflux")
+
+
+def box_tag(bx, state, tags, refine_phi, refine_phigrad, comps, tagval=1):
+    """ErrorEst on one box; tags: uint8 array [nz][ny][nx] over bx, updated in place"""
+    assert tags.dtype == np.uint8 and tags.flags.c_contiguous
+    n = len(comps)
+    ph = (C.c_double * max(n, 1))(*refine_phi)
+    gr = (C.c_double * max(n, 1))(*refine_phigrad)
+    cp = (C.c_int * max(n, 1))(*comps)
+    _ok(lib().orc_box_tag(abi.make_box(*bx), _a4(state), tags.ctypes.data, ph, gr, cp, n, tagval), "orc_box_tag")

@@ -132,3 +132,47 @@ def test_flux_register_sequence(ctx, d):
     say("flux register dir %d: bitwise %s, %d coarse cells refluxed" % (d, _same(got, want_state[0]), moved))
     assert _same(got, want_state[0])
     assert moved == 2 * np.prod([ch[q] - cl[q] + 1 for q in range(3) if q != d])
+
+
+def test_box_tag(ctx):
+    from oracle import pyoracle as po
+    rng = np.random.default_rng(31)
+    nv = ctx.nv
+    bx = ((-2, 1, 0), (13, 9, 6))
+    st = _rand(rng, tuple(bx[0][d] - 1 for d in range(3)), tuple(bx[1][d] + 1 for d in range(3)), nv)
+    phi, grad, comps = [2.5, 1e30, 3.0], [3.9, 4.2, 1e30], [0, 7, nv - 1]
+    n = [bx[1][d] - bx[0][d] + 1 for d in range(3)]
+    init = np.zeros((n[2], n[1], n[0]), np.uint8)
+    init[1, 2, 3] = 9
+    want = init.copy()
+    po.box_tag(bx, st, want, phi, grad, comps, 2)
+    dt = torch.from_numpy(init).to("cuda:0")
+    amr.box_tag(ctx, bx, _dev(st), dt, phi, grad, comps, 2)
+    ctx.sync()
+    got = dt.cpu().numpy()
+    say("box tag: %d of %d cells tagged, equal %s" % ((got == 2).sum(), got.size, np.array_equal(got, want)))
+    assert np.array_equal(got, want)
+    assert 0 < (got == 2).sum() < got.size
+
+
+def test_level_tag():
+    from oracle.pyoracle import OracleLevel, box_tag
+    c = cases.porous_reactor(n_cell=(20, 16, 12), nspec=5, nreact=2, geometry="spheres", porosity=0.6, rmin=2.0,
+                             rmax=3.0, order_hyp=5, seed=6)
+    s = c.make_state()
+    ctx = runtime.Context(c)
+    L = runtime.Level(ctx)
+    L.set_state(s)
+    O = OracleLevel(c)
+    O.set_state(s)
+    from mesoflow_amd import abi
+    phi, grad, comps = [0.5, 1e30], [1e30, 50.0], [abi.VFRAC, abi.TEMP]
+    got, cnt = amr.level_tag(L, 0.0, phi, grad, comps)
+    pad = O.fill_ghosts(0.0)
+    nx, ny, nz = c.n_cell
+    want = np.zeros((nz, ny, nx), np.uint8)
+    box_tag(((0, 0, 0), (nx - 1, ny - 1, nz - 1)), (pad, (-3, -3, -3)), want, phi, grad, comps, 1)
+    say("level tag: %d tagged (oracle %d)" % (cnt, int(want.sum())))
+    assert np.array_equal(got, want) and cnt == int(want.sum()) > 0
+    L.close()
+    ctx.close()

@@ -200,3 +200,52 @@ def test_reflux_conserves_a_two_level_update():
     bad = (Uc2.copy(), (0, 0, 0))
     po.average_down(((cl,) * 3, (ch,) * 3), (np.ascontiguousarray(Uf2), (cl * r,) * 3), bad, r)
     assert np.max(np.abs(bad[0].sum(axis=(1, 2, 3)) - M0)) > 1e-3
+
+
+def _tag(state, bx, phi, grad, comps, tagval=1, init=None):
+    n = [bx[1][d] - bx[0][d] + 1 for d in range(3)]
+    tags = np.zeros((n[2], n[1], n[0]), np.uint8) if init is None else init.copy()
+    po.box_tag(bx, state, tags, phi, grad, comps, tagval)
+    return tags
+
+
+def test_tagging_step_and_thresholds():
+    """mflo_tagging.H: a jump of h in component 2 between x = 4 and 5 tags exactly
+    the two cells beside it when h exceeds the gradient threshold (strictly), and
+    a value criterion tags where |value| is above its threshold"""
+    lo, hi = (-1, -1, -1), (10, 6, 5)
+    st = _fab(lo, hi, 4)
+    a = st[0]
+    a[2, :, :, :] = 1.0
+    a[2, :, :, 6:] = 1.0 + 0.5        # array index 6 is x = 5
+    bx = ((0, 0, 0), (9, 5, 4))
+    t = _tag(st, bx, [1e30], [0.25], [2])
+    want = np.zeros_like(t)
+    want[:, :, 4:6] = 1
+    assert np.array_equal(t, want)
+    assert not np.any(_tag(st, bx, [1e30], [0.5], [2]))       # equal to the threshold: not tagged
+    a[1, 3, 3, 3] = -2.0                                      # the cell x = y = z = 2
+    t = _tag(st, bx, [1e30, 1.5], [1e30, 1e30], [2, 1])
+    assert t.sum() == 1 and t[2, 2, 2] == 1                    # |-2| > 1.5
+    init = np.zeros_like(t)
+    init[0, 0, 0] = 7
+    t = _tag(st, bx, [1e30], [1e30], [2], tagval=3, init=init)
+    assert t[0, 0, 0] == 7 and t.sum() == 7                    # tags are only set
+
+
+def test_tagging_ghost_layer_and_directions():
+    """a jump across the box boundary (in the ghost layer) tags the edge cells,
+    and each direction's difference is seen"""
+    lo, hi = (-1, -1, -1), (4, 4, 4)
+    bx = ((0, 0, 0), (3, 3, 3))
+    for d in range(3):
+        st = _fab(lo, hi, 1)
+        sl = [slice(None)] * 4
+        sl[3 - d] = 0                                            # the ghost plane at index -1 along d
+        st[0][tuple(sl)] = 1.0
+        t = _tag(st, bx, [1e30], [0.5], [0])
+        want = np.zeros_like(t)
+        sl = [slice(None)] * 3
+        sl[2 - d] = 0
+        want[tuple(sl)] = 1
+        assert np.array_equal(t, want), d
