@@ -315,6 +315,19 @@ int mf_level_set_time(void *level, double time, double dt_prev, int istep);
 int mf_level_tag(void *level, double time, const double *refine_phi, const double *refine_phigrad, const int *comps,
                  int ntag, unsigned char *tags, int64_t *ntagged);
 
+/* The boxes of the next finer level from tags of a level (the regrid of
+ * Source/mflo_evolve.cpp:177-209 -> MakeNewGrids with amr.n_error_buf,
+ * blocking_factor, max_grid_size, grid_eff and the refinement ratio). tags:
+ * host bytes over the domain n_cell (x fastest); every tag grows by
+ * n_error_buf, tags are coarsened by blocking_factor/ratio, clustered
+ * (Berger-Rigoutsos: accept at tags/volume >= grid_eff, else cut at a hole,
+ * an inflection of the signature, or in half), refined to the fine level and
+ * chopped to max_grid_size. Boxes are in fine-level indices, disjoint,
+ * aligned to blocking_factor, and cover every buffered tag. With boxes NULL
+ * and cap 0 only *nboxes is returned. Host code: needs no device. */
+int mf_tags_to_boxes(const unsigned char *tags, const int32_t n_cell[3], int n_error_buf, int blocking_factor,
+                     int max_grid_size, double grid_eff, int ratio, mf_box *boxes, int cap, int *nboxes);
+
 /* Box list, geometry and component count of a level (boxes may be NULL to
  * query the count; cap is the capacity of boxes). */
 int mf_level_boxes(void *level, mf_box *boxes, int cap, int *nboxes, mf_geom *geom, int *nv);

@@ -44,6 +44,8 @@ SIGNATURES = {
                     C.c_int], C.c_int),
     "mf_level_tag": ([C.c_void_p, C.c_double, _P(C.c_double), _P(C.c_double), _P(C.c_int), C.c_int, C.c_void_p,
                       _P(C.c_int64)], C.c_int),
+    "mf_tags_to_boxes": ([C.c_void_p, _P(C.c_int32), C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _P(abi.Box),
+                          C.c_int, _P(C.c_int)], C.c_int),
     "mf_level_create": ([C.c_void_p, _P(abi.Box), C.c_int, _P(C.c_void_p)], C.c_int),
     "mf_level_destroy": ([C.c_void_p], C.c_int),
     "mf_level_set_state": ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),

@@ -174,5 +174,13 @@ def test_level_tag():
     box_tag(((0, 0, 0), (nx - 1, ny - 1, nz - 1)), (pad, (-3, -3, -3)), want, phi, grad, comps, 1)
     say("level tag: %d tagged (oracle %d)" % (cnt, int(want.sum())))
     assert np.array_equal(got, want) and cnt == int(want.sum()) > 0
+    # the regrid of these tags: fine boxes (ratio 2) over every tagged cell
+    boxes = amr.tags_to_boxes(got, 1, 4, 16, 0.8, 2)
+    cov = np.zeros((nz * 2, ny * 2, nx * 2), np.int32)
+    for lo, hi in boxes:
+        cov[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] += 1
+    fine = np.repeat(np.repeat(np.repeat(got.astype(bool), 2, 0), 2, 1), 2, 2)
+    say("regrid: %d fine boxes, %.1f %% of the fine domain" % (len(boxes), 100.0 * (cov > 0).mean()))
+    assert cov.max() == 1 and np.all(cov[fine] == 1)
     L.close()
     ctx.close()
