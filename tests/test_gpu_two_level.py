@@ -1,0 +1,205 @@
+"""One subcycled two-level step composed from the C-ABI on the GPU, as
+mflo::timeStep does it (Source/mflo_evolve.cpp:175-249, Advance_coupled_strang
+:251-318, the flux registers of compute_dsdt_flow :655-681): the coarse level's
+RK3 step with CrseInit at every stage, two fine substeps whose RK3 stages fill
+the fine Sborder by FillPatchTwoLevels (time interpolation of the coarse states
++ cell_cons_interp, overwritten by the fine data), FineAdd at every stage, then
+Reflux and average_down. The invariant: with periodic boundaries the composite
+total of every conserved component is unchanged to round-off, and without the
+reflux it is not. Per-box calls: mf_box_rhs (fluxes), mf_box_rk3_combine,
+mf_box_time_interp, mf_box_interp_cons, mf_fluxreg_*, mf_box_average_down;
+coarse ghost cells by mf_fill_ghosts_single_level."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch  # loaded before the library opens the device
+
+from mesoflow_amd import abi, amr, cases, runtime
+
+pytestmark = pytest.mark.gpu
+
+R = 2
+NC = 16                 # coarse cells per edge
+FLO, FHI = 4, 11        # fine region in coarse cells
+W = (1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0)   # flux weights of the three stages (sixth, sixth, two3rd)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _periodic_case(n):
+    c = cases.shocktube_n2(n_cell=(n, n, n), order_hyp=5, cfl=0.4)
+    c.geom.is_periodic[0] = 1
+    c.params.lo_bc[0] = abi.BC_PERIODIC
+    c.params.hi_bc[0] = abi.BC_PERIODIC
+    return c
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+
+def _zeros(shape):
+    return torch.zeros(shape, dtype=torch.float64, device="cuda:0")
+
+
+def _ts():
+    """torch's work done before the library's stream reads its tensors"""
+    torch.cuda.synchronize()
+
+
+def _rhs(ctx, lo, hi, sb, sb_lo, time):
+    """mf_box_rhs with face fluxes; returns dsdt and the three flux tensors"""
+    nv = ctx.nv
+    n = [hi[d] - lo[d] + 1 for d in range(3)]
+    ds = _zeros((nv, n[2], n[1], n[0]))
+    fl, fv = [], (abi.Array4 * 3)()
+    for d in range(3):
+        m = list(n)
+        m[d] += 1
+        t = _zeros((nv, m[2], m[1], m[0]))
+        fl.append(t)
+        fv[d] = abi.make_array4(t.data_ptr(), lo, [lo[q] + m[q] for q in range(3)], nv)
+    sbv = abi.make_array4(sb.data_ptr(), sb_lo, [sb_lo[q] + sb.shape[3 - q] for q in range(3)], nv)
+    dsv = abi.make_array4(ds.data_ptr(), lo, [hi[q] + 1 for q in range(3)], nv)
+    _ts()
+    runtime.check(runtime.lib().mf_box_rhs(ctx.h, abi.make_box(lo, hi), sbv, dsv, fv, float(time)), "mf_box_rhs")
+    return ds, fl
+
+
+def _combine(ctx, stage, lo, hi, s_new, src, src_lo, ds, dt):
+    nv = ctx.nv
+    end = [hi[q] + 1 for q in range(3)]
+    a = abi.make_array4(s_new.data_ptr(), lo, end, nv)
+    b = abi.make_array4(src.data_ptr(), src_lo, [src_lo[q] + src.shape[3 - q] for q in range(3)], nv)
+    c = abi.make_array4(ds.data_ptr(), lo, end, nv)
+    runtime.check(runtime.lib().mf_box_rk3_combine(ctx.h, stage, abi.make_box(lo, hi), a, b, c, float(dt)),
+                  "mf_box_rk3_combine")
+
+
+def _faces(lo, hi, d, side):
+    """face box (coarse face indices) of one side of the coarse box [lo, hi]"""
+    flo, fhi = list(lo), list(hi)
+    flo[d] = fhi[d] = lo[d] if side == 0 else hi[d] + 1
+    return tuple(flo), tuple(fhi)
+
+
+def _two_level_step(reflux):
+    cc = _periodic_case(NC)
+    cf = _periodic_case(NC * R)
+    s0 = cc.make_state()
+    y = (np.arange(NC) + 0.5) / NC
+    v = 0.2 * np.sin(2 * np.pi * y)[None, :, None] * np.ones((NC, NC, NC))
+    s0[abi.VELY] = v
+    s0[abi.RHOV] = s0[abi.DENS] * v
+    s0[abi.RHOE] += 0.5 * s0[abi.DENS] * v * v
+    ctx_c, ctx_f = runtime.Context(cc), runtime.Context(cf)
+    nv = ctx_c.nv
+    Lc = runtime.Level(ctx_c)
+    Lc.set_state(s0)
+    dxc = 1.0 / NC
+    dxf = dxc / R
+    dt = 0.5 * Lc.est_dt()
+    dtf = dt / R
+    # coarse state on the device, padded view of the level for Sborder
+    pv = abi.Array4()
+    runtime.check(runtime.lib().mf_level_state_view(Lc.h, pv), "mf_level_state_view")
+    # fine level: one box over coarse cells FLO..FHI, initialised by cell_cons_interp, coarse averaged down
+    flo, fhi = (FLO * R,) * 3, (FHI * R + R - 1,) * 3
+    nfine = fhi[0] - flo[0] + 1
+    uc_old = _dev(s0)
+    fine = _zeros((nv, nfine, nfine, nfine))
+    _ts()
+    amr.interp_cons(ctx_c, (flo, fhi), (uc_old, (0, 0, 0)), (fine, flo), R)
+    amr.average_down(ctx_c, ((FLO,) * 3, (FHI,) * 3), (fine, flo), (uc_old, (0, 0, 0)), R)
+    ctx_c.sync()
+    Lc.set_state(uc_old)
+    m0 = uc_old.sum(dim=(1, 2, 3)).cpu().numpy() * dxc ** 3
+    # flux registers on the six faces of the coarsened fine box
+    regs = {}
+    for d in range(3):
+        for side in (0, 1):
+            face = _faces((FLO,) * 3, (FHI,) * 3, d, side)
+            n = [face[1][q] - face[0][q] + 1 for q in range(3)]
+            regs[(d, side)] = (face, _zeros((nv, n[2], n[1], n[0])))
+    dA_c, dA_f = dxc * dxc, dxf * dxf
+    # coarse RK3 step (Advance_coupled_strang, only_flow)
+    dom = ((0, 0, 0), (NC - 1,) * 3)
+    t0 = 0.0
+    s_new = _zeros((nv, NC, NC, NC))
+    sb_c = _zeros((nv, NC + 6, NC + 6, NC + 6))
+    _ts()
+    for stage in (1, 2, 3):
+        Lc.fill_ghosts(t0 if stage == 1 else t0 + dt)
+        ctx_c.sync()
+        _ts()
+        # the level's padded array (ghost-filled) as this stage's Sborder
+        hip = runtime._hip_memcpy()
+        assert pv.nstride * pv.ncomp == sb_c.numel()
+        assert hip(C.c_void_p(sb_c.data_ptr()), C.c_void_p(pv.p), C.c_size_t(sb_c.numel() * 8), 3) == 0
+        ds, fl = _rhs(ctx_c, *dom, sb_c, (-3, -3, -3), t0 if stage == 1 else t0 + dt)
+        src, src_lo = (sb_c, (-3, -3, -3)) if stage == 1 else (uc_old, (0, 0, 0))
+        _combine(ctx_c, stage, *dom, s_new, src, src_lo, ds, dt)
+        for (d, side), (face, reg) in regs.items():
+            amr.fluxreg_crse_add(ctx_c, d, face, (reg, face[0]), (fl[d], (0, 0, 0)), -dt * dA_c * W[stage - 1])
+        ctx_c.sync()
+        Lc.set_state(s_new)
+    ctx_c.sync()
+    uc_new = s_new.clone()
+    # two fine substeps
+    g = 3
+    glo, ghi = tuple(q - g for q in flo), tuple(q + g for q in fhi)
+    clo = tuple(q // R - 1 for q in glo)
+    chi = tuple(q // R + 1 for q in ghi)
+    nc_t = [chi[q] - clo[q] + 1 for q in range(3)]
+    ctmp = _zeros((nv, nc_t[2], nc_t[1], nc_t[0]))
+    sb_f = _zeros((nv, nfine + 2 * g, nfine + 2 * g, nfine + 2 * g))
+    for sub in range(R):
+        tf = t0 + sub * dtf
+        f_old = fine.clone()
+        f_new = _zeros(fine.shape)
+        _ts()
+        for stage in (1, 2, 3):
+            time = tf if stage == 1 else tf + dtf
+            amr.time_interp(ctx_c, (clo, chi), (ctmp, clo), (uc_old, (0, 0, 0)), (uc_new, (0, 0, 0)), t0, t0 + dt,
+                            time)
+            amr.interp_cons(ctx_c, (glo, ghi), (ctmp, clo), (sb_f, glo), R)
+            ctx_c.sync()
+            sb_f[:, g:-g, g:-g, g:-g] = f_old if stage == 1 else f_new
+            _ts()
+            ds, fl = _rhs(ctx_f, flo, fhi, sb_f, glo, time)
+            src, src_lo = (sb_f, glo) if stage == 1 else (f_old, flo)
+            _combine(ctx_f, stage, flo, fhi, f_new, src, src_lo, ds, dtf)
+            ctx_f.sync()
+            for (d, side), (face, reg) in regs.items():
+                amr.fluxreg_fine_add(ctx_c, d, face, (reg, face[0]), (fl[d], flo), R, dtf * dA_f * W[stage - 1])
+            ctx_c.sync()
+        fine = f_new
+    # Reflux and AverageDownTo
+    if reflux:
+        for (d, side), (face, reg) in regs.items():
+            amr.fluxreg_reflux(ctx_c, d, side, face, (uc_new, (0, 0, 0)), (reg, face[0]), 1.0, dxc ** 3)
+    amr.average_down(ctx_c, ((FLO,) * 3, (FHI,) * 3), (fine, flo), (uc_new, (0, 0, 0)), R)
+    ctx_c.sync()
+    m1 = uc_new.sum(dim=(1, 2, 3)).cpu().numpy() * dxc ** 3
+    mag = uc_new.abs().sum(dim=(1, 2, 3)).cpu().numpy() * dxc ** 3
+    ok = np.all(np.isfinite(uc_new.cpu().numpy()))
+    Lc.close()
+    ctx_c.close()
+    ctx_f.close()
+    cons = list(range(5)) + list(range(abi.FLO_NVARS, nv))
+    return (m1 - m0)[cons], mag[cons], ok
+
+
+def test_subcycled_two_level_step_conserves():
+    dm, mag, ok = _two_level_step(reflux=True)
+    rel = np.abs(dm) / np.where(mag > 0, mag, 1.0)   # a component that is zero everywhere stays zero
+    say("two-level step with reflux: relative change of the totals %s" % np.array2string(rel, precision=2))
+    assert ok
+    dm0, mag0, _ = _two_level_step(reflux=False)
+    rel0 = np.abs(dm0) / np.where(mag0 > 0, mag0, 1.0)
+    say("without reflux: %s" % np.array2string(rel0, precision=2))
+    assert np.all(rel <= 1e-13)
+    assert rel0.max() > 1e3 * max(rel.max(), 1e-16)
