@@ -192,6 +192,68 @@ def read_traffic(config):
     return None
 
 
+def amr_ops_bench(ctx, lvl, reps):
+    """The two-level coupling operators and the tagging on the full level: the
+    level's state as the fine data of a ratio-2 hierarchy (coarse = its
+    average_down). Wall time of reps launches after one untimed launch, with
+    algorithmic HBM bytes: average_down reads the fine and writes the coarse
+    state; cell_cons_interp reads the coarse and writes the fine state;
+    time interpolation reads two states and writes one; tagging reads the
+    tagged components and writes one byte per cell."""
+    import torch
+    from mesoflow_amd import abi, amr, runtime
+    nx, ny, nz = ctx.case.n_cell
+    nv = ctx.nv
+    fine = torch.empty((nv, nz, ny, nx), dtype=torch.float64, device="cuda")
+    lvl.get_state(out=fine)
+    cn = (nx // 2, ny // 2, nz // 2)
+    crse = torch.zeros((nv, cn[2] + 2, cn[1] + 2, cn[0] + 2), dtype=torch.float64, device="cuda")
+    other = fine.clone()
+    dst = torch.empty_like(fine)
+    tags = torch.zeros((nz, ny, nx), dtype=torch.uint8, device="cuda")
+    pv = abi.Array4()
+    runtime.check(runtime.lib().mf_level_state_view(lvl.h, pv), "mf_level_state_view")
+    lvl.fill_ghosts(0.0)
+    torch.cuda.synchronize()
+    ctx.sync()
+    fab_bytes = nv * nx * ny * nz * 8
+    cfab_bytes = nv * cn[0] * cn[1] * cn[2] * 8
+    full = ((0, 0, 0), (nx - 1, ny - 1, nz - 1))
+    cbox = ((0, 0, 0), (cn[0] - 1, cn[1] - 1, cn[2] - 1))
+    tagc = [abi.TEMP, abi.FLO_NVARS]
+
+    def tag_call():
+        ph, gr, cp, n = amr._tag_params([1e30, 1e30], [50.0, 1e-3], tagc)
+        runtime.check(runtime.lib().mf_box_tag(ctx.h, abi.make_box(*full), pv, tags.data_ptr(), ph, gr, cp, n, 1),
+                      "mf_box_tag")
+
+    ops = {
+        "average_down": (lambda: amr.average_down(ctx, cbox, (fine, (0, 0, 0)), (crse, (-1, -1, -1)), 2),
+                         fab_bytes + cfab_bytes),
+        "interp_cons": (lambda: amr.interp_cons(ctx, full, (crse, (-1, -1, -1)), (dst, (0, 0, 0)), 2),
+                        fab_bytes + cfab_bytes),
+        "time_interp": (lambda: amr.time_interp(ctx, full, (dst, (0, 0, 0)), (fine, (0, 0, 0)), (other, (0, 0, 0)),
+                                                0.0, 1.0, 0.37), 3 * fab_bytes),
+        "tag": (tag_call, len(tagc) * nx * ny * nz * 8 + nx * ny * nz),
+    }
+    out = {"level": "%dx%dx%d, nv %d, ratio 2" % (nx, ny, nz, nv), "reps": reps}
+    for name, (fn, nbytes) in ops.items():
+        fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.sync()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[name] = {"ms": round(ms, 4), "algorithmic_GB": round(nbytes / 1e9, 3), "GB_per_s": round(gbs, 1),
+                     "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    out["tagged_cells"] = int((tags != 0).sum().item())
+    del fine, crse, other, dst, tags
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +272,8 @@ def main():
                     help="chemistry substeps of the split-mode measurement (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the discrete-adjoint measurement at 128^3 (0 = skip)")
+    ap.add_argument("--amr-reps", type=int, default=5,
+                    help="launches of each two-level coupling operator and of the tagging at full size (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -329,6 +393,10 @@ def main():
                 "OMP_NUM_THREADS", os.cpu_count())
         extra["split_chemistry"] = sc
         log("split chemistry: %s" % json.dumps(sc))
+
+    if args.amr_reps > 0:
+        extra["amr_ops"] = amr_ops_bench(ctx, lvl, args.amr_reps)
+        log("amr operators: %s" % json.dumps(extra["amr_ops"]))
 
     if args.adjoint_steps > 0:
         lvl.close()

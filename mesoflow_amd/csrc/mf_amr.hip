@@ -124,13 +124,15 @@ __global__ __launch_bounds__(kBlock) void k_time_interp(Ext b, int nv, View dst,
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_interp_cons(Ext fb, int nv, View c, View f, int r, double dfac) {
-    const int64_t tot = fb.cells * nv;
+// one thread per coarse cell and component: the limited slopes once, then the
+// r^3 children that lie in the fine box fb
+__global__ __launch_bounds__(kBlock) void k_interp_cons(Ext cb, int nv, View c, View f, int r, double dfac, int flo0,
+                                                        int flo1, int flo2, int fhi0, int fhi1, int fhi2) {
+    const int64_t tot = cb.cells * nv;
     const double rr = (double)r;
     for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < tot; q += (int64_t)gridDim.x * kBlock) {
-        int i, j, k, n;
-        fb.at(q, i, j, k, n);
-        const int ic = floor_div(i, r), jc = floor_div(j, r), kc = floor_div(k, r);
+        int ic, jc, kc, n;
+        cb.at(q, ic, jc, kc, n);
         const double u = c(ic, jc, kc, n);
         double sx = mc_slope(c(ic - 1, jc, kc, n), u, c(ic + 1, jc, kc, n));
         double sy = mc_slope(c(ic, jc - 1, kc, n), u, c(ic, jc + 1, kc, n));
@@ -152,10 +154,22 @@ __global__ __launch_bounds__(kBlock) void k_interp_cons(Ext fb, int nv, View c, 
         sx *= alpha;
         sy *= alpha;
         sz *= alpha;
-        const double xo = ((double)(i - ic * r) + 0.5) / rr - 0.5;
-        const double yo = ((double)(j - jc * r) + 0.5) / rr - 0.5;
-        const double zo = ((double)(k - kc * r) + 0.5) / rr - 0.5;
-        f(i, j, k, n) = ((u + rn(xo * sx)) + rn(yo * sy)) + rn(zo * sz);
+        for (int kr = 0; kr < r; ++kr) {
+            const int k = kc * r + kr;
+            if (k < flo2 || k > fhi2) continue;
+            const double zt = rn((((double)kr + 0.5) / rr - 0.5) * sz);
+            for (int jr = 0; jr < r; ++jr) {
+                const int j = jc * r + jr;
+                if (j < flo1 || j > fhi1) continue;
+                const double yt = rn((((double)jr + 0.5) / rr - 0.5) * sy);
+                for (int ir = 0; ir < r; ++ir) {
+                    const int i = ic * r + ir;
+                    if (i < flo0 || i > fhi0) continue;
+                    const double xt = rn((((double)ir + 0.5) / rr - 0.5) * sx);
+                    f(i, j, k, n) = ((u + xt) + yt) + zt;
+                }
+            }
+        }
     }
 }
 
@@ -348,9 +362,12 @@ int mf_box_interp_cons(void *ctx, mf_box fbx, mf_array4 crse, mf_array4 fine, in
         chi[d] = ((h >= 0) ? h / ratio : -((-h + ratio - 1) / ratio)) + 1;
     }
     if (!holds(fine, fbx.lo, fbx.hi, nv) || !holds(crse, clo, chi, nv)) return MF_ERR_BAD_ARG;
-    const Ext e = ext_of(fbx);
+    mf_box cbx;  // coarsen(fbx)
+    for (int d = 0; d < 3; ++d) { cbx.lo[d] = clo[d] + 1; cbx.hi[d] = chi[d] - 1; }
+    const Ext e = ext_of(cbx);
     hipLaunchKernelGGL(k_interp_cons, dim3(grid_for(e.cells * nv)), dim3(kBlock), 0, st, e, nv, view_of(crse),
-                       view_of(fine), ratio, (double)(ratio - 1) / (double)(2 * ratio));
+                       view_of(fine), ratio, (double)(ratio - 1) / (double)(2 * ratio), fbx.lo[0], fbx.lo[1],
+                       fbx.lo[2], fbx.hi[0], fbx.hi[1], fbx.hi[2]);
     return hipGetLastError() == hipSuccess ? MF_OK : MF_ERR_HIP;
 }
 
