@@ -214,6 +214,21 @@ int mf_box_time_interp(void *ctx, mf_box bx, mf_array4 dst, mf_array4 s_old, mf_
  * children is the coarse value). crse must hold coarsen(fbx) grown by 1. */
 int mf_box_interp_cons(void *ctx, mf_box fbx, mf_array4 crse, mf_array4 fine, int ratio);
 
+/* FillPatchTwoLevels (Source/mflo_gridwork.cpp:147-165) for the fine box fbx:
+ * sborder over fbx grown by MF_NGROW gets, in this order, the coarse data at
+ * `time` (mf_box_time_interp of crse_old/crse_new, which must hold the coarse
+ * cells under the region grown by one: ghost-filled coarse arrays) mapped by
+ * mf_box_interp_cons, then the fine data of every fine box (fboxes/fdata, the
+ * fine level's valid data at `time`, fbx's own included) and of its periodic
+ * images where they cover the region. ctx is the fine level's context (its
+ * geometry gives the domain and periodicity). A region that leaves the domain
+ * in a non-periodic direction is MF_ERR_UNSUPPORTED (the physical boundary
+ * fill of the fine level is not applied here). Allocates and frees a scratch
+ * array for the coarse data and waits for the context's stream. */
+int mf_fill_patch_two_levels(void *ctx, mf_box fbx, mf_array4 sborder, const mf_box *fboxes, const mf_array4 *fdata,
+                             int nfine, mf_array4 crse_old, mf_array4 crse_new, double t_old, double t_new, double time,
+                             int ratio);
+
 /* FluxRegister on one face box (coarse face indices; along dir lo == hi is the
  * face index) of the boundary of a coarsened fine box; reg is a face-centred
  * view of it.

@@ -34,6 +34,20 @@ def interp_cons(ctx, fbx, crse, fine, ratio):
                   "mf_box_interp_cons")
 
 
+def fill_patch_two_levels(ctx, fbx, sborder, fine_fabs, crse_old, crse_new, t_old, t_new, time, ratio):
+    """FillPatchTwoLevels for one fine box: sborder (fbx grown by 3) from the
+    coarse states at time and the fine fabs [(box, fab)] (ctx: the fine level's)"""
+    nb = len(fine_fabs)
+    boxes = (abi.Box * max(nb, 1))()
+    views = (abi.Array4 * max(nb, 1))()
+    for q, (bx, fab) in enumerate(fine_fabs):
+        boxes[q] = abi.make_box(*bx)
+        views[q] = _a4(fab)
+    runtime.check(runtime.lib().mf_fill_patch_two_levels(ctx.h, abi.make_box(*fbx), _a4(sborder), boxes, views, nb,
+                                                         _a4(crse_old), _a4(crse_new), float(t_old), float(t_new),
+                                                         float(time), ratio), "mf_fill_patch_two_levels")
+
+
 def fluxreg_crse_add(ctx, d, face, reg, cflux, scale):
     runtime.check(runtime.lib().mf_fluxreg_crse_add(ctx.h, d, abi.make_box(*face), _a4(reg), _a4(cflux),
                                                     float(scale)), "mf_fluxreg_crse_add")

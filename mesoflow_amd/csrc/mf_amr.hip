@@ -23,6 +23,7 @@
 using namespace mfd;
 
 int mf_ctx_stream(void *ctx, hipStream_t *stream, int *nv);  // mf_capi.hip
+int mf_ctx_geom(void *ctx, mf_geom *geom);                   // mf_capi.hip
 
 namespace {
 
@@ -214,6 +215,16 @@ __global__ __launch_bounds__(kBlock) void k_reflux(Ext fb, int nv, View s, View 
     }
 }
 
+// dst = src shifted by sh (a periodic image) over the region
+__global__ __launch_bounds__(kBlock) void k_copy_shift(Ext b, int nv, View dst, View src, int sh0, int sh1, int sh2) {
+    const int64_t tot = b.cells * nv;
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < tot; q += (int64_t)gridDim.x * kBlock) {
+        int i, j, k, n;
+        b.at(q, i, j, k, n);
+        dst(i, j, k, n) = src(i + sh0, j + sh1, k + sh2, n);
+    }
+}
+
 constexpr int kMaxTagVars = 16;
 struct TagVars {
     int n;
@@ -258,6 +269,78 @@ int tag_vars(const double *phi, const double *grad, const int *comps, int ntag, 
 int mf_level_ctx(void *level, void **ctx);  // mf_capi.hip
 
 extern "C" {
+
+int mf_fill_patch_two_levels(void *ctx, mf_box fbx, mf_array4 sborder, const mf_box *fboxes, const mf_array4 *fdata,
+                             int nfine, mf_array4 crse_old, mf_array4 crse_new, double t_old, double t_new, double time,
+                             int ratio) {
+    hipStream_t st;
+    int nv;
+    if (!ctx || ratio < 1 || ratio > 8 || !valid_box(fbx) || nfine < 0 || (nfine > 0 && (!fboxes || !fdata)))
+        return MF_ERR_BAD_ARG;
+    int rc = mf_ctx_stream(ctx, &st, &nv);
+    if (rc) return rc;
+    mf_geom g;
+    rc = mf_ctx_geom(ctx, &g);
+    if (rc) return rc;
+    mf_box gb;  // the region filled: fbx grown by the ghost width
+    for (int d = 0; d < 3; ++d) {
+        gb.lo[d] = fbx.lo[d] - MF_NGROW;
+        gb.hi[d] = fbx.hi[d] + MF_NGROW;
+        if (!g.is_periodic[d] && (gb.lo[d] < 0 || gb.hi[d] >= g.n_cell[d])) return MF_ERR_UNSUPPORTED;
+        if (g.n_cell[d] % ratio) return MF_ERR_BAD_ARG;
+    }
+    if (!holds(sborder, gb.lo, gb.hi, nv)) return MF_ERR_BAD_ARG;
+    for (int b = 0; b < nfine; ++b)
+        if (!valid_box(fboxes[b]) || !holds(fdata[b], fboxes[b].lo, fboxes[b].hi, nv)) return MF_ERR_BAD_ARG;
+    // 1. coarse data at the fill time on coarsen(gb) grown by one
+    mf_box cb;
+    for (int d = 0; d < 3; ++d) {
+        cb.lo[d] = ((gb.lo[d] >= 0) ? gb.lo[d] / ratio : -((-gb.lo[d] + ratio - 1) / ratio)) - 1;
+        cb.hi[d] = ((gb.hi[d] >= 0) ? gb.hi[d] / ratio : -((-gb.hi[d] + ratio - 1) / ratio)) + 1;
+    }
+    if (!holds(crse_old, cb.lo, cb.hi, nv) || !holds(crse_new, cb.lo, cb.hi, nv)) return MF_ERR_BAD_ARG;
+    const Ext ce = ext_of(cb);
+    double *tmp = nullptr;
+    if (hipMalloc(&tmp, (size_t)ce.cells * nv * sizeof(double)) != hipSuccess) return MF_ERR_HIP;
+    mf_array4 ct{};
+    ct.p = tmp;
+    ct.jstride = ce.e[0];
+    ct.kstride = (int64_t)ce.e[0] * ce.e[1];
+    ct.nstride = ce.cells;
+    for (int d = 0; d < 3; ++d) { ct.begin[d] = cb.lo[d]; ct.end[d] = cb.hi[d] + 1; }
+    ct.ncomp = nv;
+    rc = mf_box_time_interp(ctx, cb, ct, crse_old, crse_new, t_old, t_new, time);
+    // 2. the conservative interpolation over the whole region
+    if (!rc) rc = mf_box_interp_cons(ctx, gb, ct, sborder, ratio);
+    // 3. fine data wherever a fine box (or its periodic image) covers the region
+    for (int b = 0; !rc && b < nfine; ++b) {
+        int span[3][2];
+        for (int d = 0; d < 3; ++d) {
+            span[d][0] = g.is_periodic[d] ? -1 : 0;
+            span[d][1] = g.is_periodic[d] ? 1 : 0;
+        }
+        for (int pk = span[2][0]; pk <= span[2][1]; ++pk)
+            for (int pj = span[1][0]; pj <= span[1][1]; ++pj)
+                for (int pi = span[0][0]; pi <= span[0][1]; ++pi) {
+                    const int sh[3] = {pi * g.n_cell[0], pj * g.n_cell[1], pk * g.n_cell[2]};
+                    mf_box x;
+                    bool empty = false;
+                    for (int d = 0; d < 3; ++d) {  // region cells whose shifted position lies in fine box b
+                        x.lo[d] = std::max(gb.lo[d], fboxes[b].lo[d] + sh[d]);
+                        x.hi[d] = std::min(gb.hi[d], fboxes[b].hi[d] + sh[d]);
+                        if (x.hi[d] < x.lo[d]) empty = true;
+                    }
+                    if (empty) continue;
+                    const Ext xe = ext_of(x);
+                    hipLaunchKernelGGL(k_copy_shift, dim3(grid_for(xe.cells * nv)), dim3(kBlock), 0, st, xe, nv,
+                                       view_of(sborder), view_of(fdata[b]), -sh[0], -sh[1], -sh[2]);
+                    if (hipGetLastError() != hipSuccess) rc = MF_ERR_HIP;
+                }
+    }
+    hipStreamSynchronize(st);
+    hipFree(tmp);
+    return rc;
+}
 
 int mf_box_tag(void *ctx, mf_box bx, mf_array4 state, unsigned char *tags, const double *refine_phi,
                const double *refine_phigrad, const int *comps, int ntag, int tagval) {

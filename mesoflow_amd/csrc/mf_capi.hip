@@ -2009,3 +2009,11 @@ int mf_level_ctx(void *level, void **ctx) {
     *ctx = L->c;
     return MF_OK;
 }
+
+// geometry of a context, for mf_amr.hip
+int mf_ctx_geom(void *ctx, mf_geom *geom) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c || !geom) return MF_ERR_BAD_ARG;
+    *geom = c->geom;
+    return MF_OK;
+}

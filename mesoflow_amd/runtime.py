@@ -36,6 +36,8 @@ SIGNATURES = {
     "mf_box_time_interp": ([C.c_void_p, abi.Box, abi.Array4, abi.Array4, abi.Array4, C.c_double, C.c_double,
                             C.c_double], C.c_int),
     "mf_box_interp_cons": ([C.c_void_p, abi.Box, abi.Array4, abi.Array4, C.c_int], C.c_int),
+    "mf_fill_patch_two_levels": ([C.c_void_p, abi.Box, abi.Array4, _P(abi.Box), _P(abi.Array4), C.c_int, abi.Array4,
+                                  abi.Array4, C.c_double, C.c_double, C.c_double, C.c_int], C.c_int),
     "mf_fluxreg_crse_add": ([C.c_void_p, C.c_int, abi.Box, abi.Array4, abi.Array4, C.c_double], C.c_int),
     "mf_fluxreg_fine_add": ([C.c_void_p, C.c_int, abi.Box, abi.Array4, abi.Array4, C.c_int, C.c_double], C.c_int),
     "mf_fluxreg_reflux": ([C.c_void_p, C.c_int, C.c_int, abi.Box, abi.Array4, abi.Array4, C.c_double, C.c_double],

@@ -184,3 +184,48 @@ def test_level_tag():
     assert cov.max() == 1 and np.all(cov[fine] == 1)
     L.close()
     ctx.close()
+
+
+def test_fill_patch_two_levels():
+    """four fine boxes of a periodic 32^3 fine level (ratio 2), two of them
+    neighbours across the periodic seam: each box's Sborder against the oracle's time interpolation +
+    cell_cons_interp followed by copies of the fine data and its periodic images"""
+    from oracle import pyoracle as po
+    c = cases.shocktube_n2(n_cell=(32, 32, 32))
+    c.geom.is_periodic[0] = 1
+    c.params.lo_bc[0] = c.params.hi_bc[0] = 0
+    fctx = runtime.Context(c)
+    nv = fctx.nv
+    rng = np.random.default_rng(41)
+    co = _rand(rng, (-3, -3, -3), (18, 18, 18), nv)
+    cn = _rand(rng, (-3, -3, -3), (18, 18, 18), nv)
+    boxes = [((8, 8, 8), (15, 23, 23)), ((16, 8, 8), (23, 23, 23)), ((0, 0, 26), (5, 7, 31)), ((26, 0, 0), (31, 7, 5))]
+    fabs = [_rand(rng, *b, nv) for b in boxes]
+    dfabs = [(b, _dev(f)) for b, f in zip(boxes, fabs)]
+    dco, dcn = _dev(co), _dev(cn)
+    N = 32
+    for q, (lo, hi) in enumerate(boxes):
+        glo, ghi = tuple(v - 3 for v in lo), tuple(v + 3 for v in hi)
+        clo, chi = tuple(v // 2 - 1 for v in glo), tuple(v // 2 + 1 for v in ghi)
+        ct = _rand(rng, clo, chi, nv)
+        po.time_interp((clo, chi), ct, co, cn, 0.0, 1.0, 0.25)
+        want = _rand(rng, glo, ghi, nv)
+        po.interp_cons((glo, ghi), ct, want, 2)
+        w = want[0]
+        for (blo, bhi), (fa, _) in zip(boxes, fabs):
+            for sh in [(a, b_, c_) for a in (-N, 0, N) for b_ in (-N, 0, N) for c_ in (-N, 0, N)]:
+                xl = [max(glo[d], blo[d] + sh[d]) for d in range(3)]
+                xh = [min(ghi[d], bhi[d] + sh[d]) for d in range(3)]
+                if any(xh[d] < xl[d] for d in range(3)):
+                    continue
+                w[:, xl[2] - glo[2]:xh[2] - glo[2] + 1, xl[1] - glo[1]:xh[1] - glo[1] + 1,
+                  xl[0] - glo[0]:xh[0] - glo[0] + 1] = \
+                    fa[:, xl[2] - sh[2] - blo[2]:xh[2] - sh[2] - blo[2] + 1,
+                       xl[1] - sh[1] - blo[1]:xh[1] - sh[1] - blo[1] + 1,
+                       xl[0] - sh[0] - blo[0]:xh[0] - sh[0] - blo[0] + 1]
+        dsb = _dev(_rand(rng, glo, ghi, nv))
+        amr.fill_patch_two_levels(fctx, (lo, hi), dsb, dfabs, dco, dcn, 0.0, 1.0, 0.25, 2)
+        got = _host(dsb, fctx)
+        say("fill patch two levels, box %d: bitwise %s" % (q, _same(got, w)))
+        assert _same(got, w)
+    fctx.close()

@@ -2,12 +2,12 @@
 mflo::timeStep does it (Source/mflo_evolve.cpp:175-249, Advance_coupled_strang
 :251-318, the flux registers of compute_dsdt_flow :655-681): the coarse level's
 RK3 step with CrseInit at every stage, two fine substeps whose RK3 stages fill
-the fine Sborder by FillPatchTwoLevels (time interpolation of the coarse states
-+ cell_cons_interp, overwritten by the fine data), FineAdd at every stage, then
+the fine Sborder by mf_fill_patch_two_levels (time interpolation of the coarse
+states + cell_cons_interp, overwritten by the fine data), FineAdd at every stage, then
 Reflux and average_down. The invariant: with periodic boundaries the composite
 total of every conserved component is unchanged to round-off, and without the
 reflux it is not. Per-box calls: mf_box_rhs (fluxes), mf_box_rk3_combine,
-mf_box_time_interp, mf_box_interp_cons, mf_fluxreg_*, mf_box_average_down;
+mf_fill_patch_two_levels, mf_fluxreg_*, mf_box_interp_cons, mf_box_average_down;
 coarse ghost cells by mf_fill_ghosts_single_level."""
 import ctypes as C
 
@@ -150,11 +150,7 @@ def _two_level_step(reflux):
     uc_new = s_new.clone()
     # two fine substeps
     g = 3
-    glo, ghi = tuple(q - g for q in flo), tuple(q + g for q in fhi)
-    clo = tuple(q // R - 1 for q in glo)
-    chi = tuple(q // R + 1 for q in ghi)
-    nc_t = [chi[q] - clo[q] + 1 for q in range(3)]
-    ctmp = _zeros((nv, nc_t[2], nc_t[1], nc_t[0]))
+    glo = tuple(q - g for q in flo)
     sb_f = _zeros((nv, nfine + 2 * g, nfine + 2 * g, nfine + 2 * g))
     for sub in range(R):
         tf = t0 + sub * dtf
@@ -163,12 +159,10 @@ def _two_level_step(reflux):
         _ts()
         for stage in (1, 2, 3):
             time = tf if stage == 1 else tf + dtf
-            amr.time_interp(ctx_c, (clo, chi), (ctmp, clo), (uc_old, (0, 0, 0)), (uc_new, (0, 0, 0)), t0, t0 + dt,
-                            time)
-            amr.interp_cons(ctx_c, (glo, ghi), (ctmp, clo), (sb_f, glo), R)
-            ctx_c.sync()
-            sb_f[:, g:-g, g:-g, g:-g] = f_old if stage == 1 else f_new
-            _ts()
+            cur = f_old if stage == 1 else f_new
+            amr.fill_patch_two_levels(ctx_f, (flo, fhi), (sb_f, glo), [((flo, fhi), (cur, flo))], (uc_old, (0, 0, 0)),
+                                      (uc_new, (0, 0, 0)), t0, t0 + dt, time, R)
+            ctx_f.sync()
             ds, fl = _rhs(ctx_f, flo, fhi, sb_f, glo, time)
             src, src_lo = (sb_f, glo) if stage == 1 else (f_old, flo)
             _combine(ctx_f, stage, flo, fhi, f_new, src, src_lo, ds, dtf)
