@@ -197,3 +197,57 @@ def test_subcycled_two_level_step_conserves():
     say("without reflux: %s" % np.array2string(rel0, precision=2))
     assert np.all(rel <= 1e-13)
     assert rel0.max() > 1e3 * max(rel.max(), 1e-16)
+
+
+def test_regrid_cycle_remakes_the_fine_level():
+    """regrid of a two-level hierarchy (timeStep -> regrid -> RemakeLevel /
+    MakeNewLevelFromCoarse, Source/mflo_evolve.cpp:177-209,
+    Source/mflo_gridwork.cpp:21-66): tags of the coarse level (mf_level_tag),
+    new fine boxes (mf_tags_to_boxes), the new fine data by FillPatch of each
+    new box (mf_fill_patch_two_levels): old fine data where the old fine level
+    covered, conservative interpolation of the coarse level elsewhere"""
+    from oracle import pyoracle as po
+    cc = _periodic_case(NC)
+    cf = _periodic_case(NC * R)
+    s0 = cc.make_state()
+    ctx_c, ctx_f = runtime.Context(cc), runtime.Context(cf)
+    nv = ctx_c.nv
+    Lc = runtime.Level(ctx_c)
+    Lc.set_state(s0)
+    tags, ntag = amr.level_tag(Lc, 0.0, [1e30], [0.1], [abi.DENS])
+    new_boxes = amr.tags_to_boxes(tags, 1, 4, 16, 0.7, R)
+    say("regrid: %d tagged coarse cells -> %d fine boxes" % (ntag, len(new_boxes)))
+    assert ntag > 0 and new_boxes
+    # the old fine level: one box of interpolated coarse data plus a perturbation
+    old_box = ((12, 8, 8), (19, 23, 23))
+    n_old = [old_box[1][d] - old_box[0][d] + 1 for d in range(3)]
+    rng = np.random.default_rng(5)
+    old = rng.standard_normal((nv, n_old[2], n_old[1], n_old[0]))
+    dold = _dev(old)
+    pv = abi.Array4()
+    runtime.check(runtime.lib().mf_level_state_view(Lc.h, pv), "mf_level_state_view")
+    Lc.fill_ghosts(0.0)
+    ctx_c.sync()
+    crse = Lc.padded_state()          # ghost-filled coarse state, first cell (-3, -3, -3)
+    dcrse = _dev(crse)
+    for lo, hi in new_boxes:
+        glo, ghi = tuple(v - 3 for v in lo), tuple(v + 3 for v in hi)
+        n = [ghi[d] - glo[d] + 1 for d in range(3)]
+        sb = _zeros((nv, n[2], n[1], n[0]))
+        _ts()
+        amr.fill_patch_two_levels(ctx_f, (lo, hi), (sb, glo), [(old_box, (dold, old_box[0]))], (dcrse, (-3, -3, -3)),
+                                  (dcrse, (-3, -3, -3)), 0.0, 1.0, 0.0, R)
+        ctx_f.sync()
+        got = sb.cpu().numpy()[:, 3:-3, 3:-3, 3:-3]
+        want = np.zeros((nv, hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1))
+        po.interp_cons((lo, hi), (np.ascontiguousarray(crse), (-3, -3, -3)), (want, lo), R)
+        xl = [max(lo[d], old_box[0][d]) for d in range(3)]
+        xh = [min(hi[d], old_box[1][d]) for d in range(3)]
+        if all(xh[d] >= xl[d] for d in range(3)):
+            want[:, xl[2] - lo[2]:xh[2] - lo[2] + 1, xl[1] - lo[1]:xh[1] - lo[1] + 1, xl[0] - lo[0]:xh[0] - lo[0] + 1] = \
+                old[:, xl[2] - old_box[0][2]:xh[2] - old_box[0][2] + 1, xl[1] - old_box[0][1]:xh[1] - old_box[0][1] + 1,
+                    xl[0] - old_box[0][0]:xh[0] - old_box[0][0] + 1]
+        assert np.array_equal(got, want), (lo, hi)
+    Lc.close()
+    ctx_c.close()
+    ctx_f.close()
