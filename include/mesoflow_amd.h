@@ -189,6 +189,15 @@ int mf_box_rhs(void *ctx, mf_box bx, mf_array4 sborder, mf_array4 dsdt,
 int mf_box_rk3_combine(void *ctx, int stage, mf_box bx, mf_array4 s_new,
                        mf_array4 src, mf_array4 dsdt, double dt);
 
+/* The per-cell passes after a level's RK3 flow step, on one box in place
+ * (flags, applied in this order): MF_POST_CHEMISTRY Advance_chemistry_implicit
+ * over dt (Source/mflo_evolve.cpp:470-493, 684-720: SSPRK3 on the species
+ * source), MF_POST_PRIMITIVES update_primitive_vars (:114-171; a negative
+ * pressure with do_ns is reported as MF_ERR_NEG_PRESSURE by mf_ctx_sync),
+ * MF_POST_CLIP clip_neg_speciesconc (:521-540). state must hold bx. */
+enum { MF_POST_CHEMISTRY = 1, MF_POST_PRIMITIVES = 2, MF_POST_CLIP = 4 };
+int mf_box_post(void *ctx, mf_box bx, mf_array4 state, double dt, int flags);
+
 /* ------------------------------------------------------------------------ */
 /* Two-level coupling: the AMReX operations with which mesoflow couples a     */
 /* fine level (refinement ratio r) to its coarse level, as per-box device     */

@@ -39,6 +39,7 @@ PLUGIN_CATALYST_CYLINDER = 2
 PLUGIN_POROUS_ARRHENIUS = 3
 
 MODE_PER_BOX, MODE_BATCHED, MODE_GRAPH = 0, 1, 2
+MF_POST_CHEMISTRY, MF_POST_PRIMITIVES, MF_POST_CLIP = 1, 2, 4
 
 
 class Array4(C.Structure):

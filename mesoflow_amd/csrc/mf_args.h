@@ -180,6 +180,16 @@ struct ChemSplitArgs {
     int *err;
 };
 
+// mf_box_post: chemistry, primitive update and clip on one box, in place
+struct BoxPostArgs {
+    const Tab *tab;
+    View s;
+    int lo[3], e[3];
+    double dt;
+    int chem, prims, clip, conj_ht, do_ns;
+    int *err;
+};
+
 // Host launchers of one compiled plugin family.
 struct KernelSet {
     int ns;
@@ -195,6 +205,7 @@ struct KernelSet {
     void (*est)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
     void (*post)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
     void (*chem_split)(const ChemSplitArgs &, hipStream_t);
+    void (*box_post)(const BoxPostArgs &, hipStream_t);
     const AdjSet *adj;         // discrete adjoint (null: not compiled for this family)
 };
 

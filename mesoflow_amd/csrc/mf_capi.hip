@@ -1301,6 +1301,29 @@ int mf_box_rk3_combine(void *ctx, int stage, mf_box bx, mf_array4 s_new, mf_arra
     return MF_OK;
 }
 
+int mf_box_post(void *ctx, mf_box bx, mf_array4 state, double dt, int flags) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c || (flags & ~(MF_POST_CHEMISTRY | MF_POST_PRIMITIVES | MF_POST_CLIP))) return MF_ERR_BAD_ARG;
+    for (int d = 0; d < 3; ++d)
+        if (bx.hi[d] < bx.lo[d]) return MF_ERR_BAD_ARG;
+    if (!covers(state, bx, c->nv)) return MF_ERR_BAD_ARG;
+    if (!flags) return MF_OK;
+    BoxPostArgs a;
+    a.tab = c->d_tab;
+    a.s = from_array4(state);
+    for (int d = 0; d < 3; ++d) { a.lo[d] = bx.lo[d]; a.e[d] = bx.hi[d] - bx.lo[d] + 1; }
+    a.dt = dt;
+    a.chem = (flags & MF_POST_CHEMISTRY) ? 1 : 0;
+    a.prims = (flags & MF_POST_PRIMITIVES) ? 1 : 0;
+    a.clip = (flags & MF_POST_CLIP) ? 1 : 0;
+    a.conj_ht = c->prm.conj_ht;
+    a.do_ns = c->prm.do_ns;
+    a.err = c->d_err;
+    c->ks->box_post(a, c->stream);
+    HIPCHK(hipGetLastError());
+    return MF_OK;
+}
+
 // ----- level forms -------------------------------------------------------------
 int mf_level_create(void *ctx, const mf_box *boxes, int nboxes, void **level_out) {
     Ctx *c = (Ctx *)ctx;

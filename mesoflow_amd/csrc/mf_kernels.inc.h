@@ -979,6 +979,49 @@ __global__ __launch_bounds__(256) void k_post(ResArgs r, Work w) {
     block_reduce_store<NV>(sums, mws, mD, r.partials + (size_t)(w.t0 + blockIdx.x) * (NV + 2));
 }
 
+// mf_box_post: the per-cell work of k_post without the residual and dt
+// partials, on one box view in place (the per-box form of
+// Advance_chemistry_implicit, update_primitive_vars, clip_neg_speciesconc)
+template <class P>
+__global__ __launch_bounds__(256) void k_box_post(BoxPostArgs a) {
+    using N = Num<P>;
+    constexpr int NS = P::NS, NV = N::NV;
+    const Tab &T = *a.tab;
+    const int64_t nc = (int64_t)a.e[0] * a.e[1] * a.e[2];
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nc; q += (int64_t)gridDim.x * blockDim.x) {
+        const int i = a.lo[0] + (int)(q % a.e[0]);
+        const int64_t rr = q / a.e[0];
+        const int j = a.lo[1] + (int)(rr % a.e[1]);
+        const int k = a.lo[2] + (int)(rr / a.e[1]);
+        double S[NV];
+#pragma unroll
+        for (int n = 0; n < NV; ++n) S[n] = a.s(i, j, k, n);
+        if (a.chem) chemistry_rk3<P>(T, S, a.dt);
+        if (a.prims) {
+            if (S[VFRAC] == kOne) {
+                double u[N::NC], p[NCV];
+#pragma unroll
+                for (int c = 0; c < NCV; ++c) u[c] = S[c];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) u[NCV + s] = S[FLO + s];
+                N::cons_to_prim(T, u, p);
+#pragma unroll
+                for (int c = 0; c < NCV; ++c) S[DENS + c] = p[c];
+                S[TEMP] = P::get_t_from_rpc(T, S[DENS], S[PRES], u + NCV);
+            } else if (a.conj_ht && S[VFRAC] == kZero) {
+                S[TEMP] = P::get_solid_t_from_rhoe(T, S[RHOE]);
+            }
+            if (a.do_ns && S[PRES] < 0) raise_err(a.err, MF_ERR_NEG_PRESSURE);
+        }
+        if (a.clip) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) if (S[FLO + s] < 0.0) S[FLO + s] = 0.0;
+        }
+#pragma unroll
+        for (int n = 0; n < NV; ++n) a.s(i, j, k, n) = S[n];
+    }
+}
+
 // EstTimeStep partials on the current state (first step of a run)
 template <class P>
 __global__ __launch_bounds__(256) void k_est(ResArgs r, Work w) {
@@ -1592,6 +1635,11 @@ struct Launch {
         const int64_t g = std::min<int64_t>((ncell + 255) / 256, 8192);
         hipLaunchKernelGGL(k_chem_split<P>, dim3((unsigned)std::max<int64_t>(g, 1)), dim3(256), 0, st, a);
     }
+    static void box_post(const BoxPostArgs &a, hipStream_t st) {
+        const int64_t nc = (int64_t)a.e[0] * a.e[1] * a.e[2];
+        const int64_t g = std::min<int64_t>((nc + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_box_post<P>, dim3((unsigned)std::max<int64_t>(g, 1)), dim3(256), 0, st, a);
+    }
     static KernelSet set(const AdjSet *adj = nullptr) {
         KernelSet k;
         k.ns = P::NS;
@@ -1607,6 +1655,7 @@ struct Launch {
         k.est = &est;
         k.post = &post;
         k.chem_split = &chem_split;
+        k.box_post = &box_post;
         return k;
     }
 };

@@ -1415,6 +1415,65 @@ static LevelBase *make_level(const mf_plugin_params *tab, const mf_params *prm, 
     return L;
 }
 
+// The per-cell work of the level passes after the RK3 flow step, on one box:
+// Advance_chemistry_implicit (the SSPRK3 tableau of Level::advance_chemistry),
+// update_primitive_vars (Source/mflo_evolve.cpp:114-171) and
+// clip_neg_speciesconc (:521-540), in that order, each if asked for.
+template <class P>
+static int box_post(const P &pl, const Box &B, A4 s, double dt, int chem, int prims, int clip, int conj_ht,
+                    int do_ns) {
+    const int NS = P::NS, NV = Ops<P>::NV;
+    Ops<P> op(pl);
+    bool neg = false;
+    const double a[3][2] = {{0, 0}, {1.0, 0}, {0.25, 0.25}};
+    const double w[3] = {1. / 6., 1. / 6., 2. / 3.};
+    for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+        for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+            for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+                double S[FLO_NVARS + 16];
+                for (int c = 0; c < NV; ++c) S[c] = s(i, j, k, c);
+                if (chem) {
+                    double Sst[FLO_NVARS + 16], F[3][FLO_NVARS + 16];
+                    for (int st = 0; st < 3; ++st) {
+                        for (int c = 0; c < NV; ++c) Sst[c] = S[c];
+                        for (int jj = 0; jj < st; ++jj) {
+                            double coef = dt * a[st][jj];
+                            for (int c = 0; c < NV; ++c) Sst[c] += coef * F[jj][c];
+                        }
+                        double src[FLO_NVARS + 16];
+                        for (int c = 0; c < NV; ++c) src[c] = 0.0;
+                        pl.spec_source(Sst, src);
+                        for (int c = 0; c < NV; ++c) F[st][c] = 0.0 + src[c];
+                    }
+                    double Sn[FLO_NVARS + 16];
+                    for (int c = 0; c < NV; ++c) Sn[c] = S[c];
+                    for (int st = 0; st < 3; ++st) {
+                        double coef = dt * w[st];
+                        for (int c = 0; c < NV; ++c) Sn[c] += coef * F[st][c];
+                    }
+                    for (int c = 0; c < NV; ++c) S[c] = Sn[c];
+                }
+                if (prims) {
+                    if (S[VFRAC] == one) {
+                        double u[NCVARS + 16], p[NCVARS];
+                        for (int c = 0; c < NCVARS; c++) u[c] = S[c];
+                        for (int c = 0; c < NS; c++) u[c + NCVARS] = S[c + FLO_NVARS];
+                        op.cons_to_prim(u, p);
+                        for (int c = 0; c < NCVARS; c++) S[c + DENS] = p[c];
+                        S[TEMP] = pl.get_t_from_rpc(S[DENS], S[PRES], u + NCVARS);
+                    } else if (conj_ht && S[VFRAC] == zeroval) {
+                        S[TEMP] = pl.get_solid_t_from_rhoe(S[RHOE]);
+                    }
+                    if (S[PRES] < 0 && do_ns) neg = true;
+                }
+                if (clip)
+                    for (int sp = 0; sp < NS; sp++)
+                        if (S[FLO_NVARS + sp] < 0.0) S[FLO_NVARS + sp] = 0.0;
+                for (int c = 0; c < NV; ++c) s(i, j, k, c) = S[c];
+            }
+    return neg ? MF_ERR_NEG_PRESSURE : MF_OK;
+}
+
 // dispatch over the compiled plugin families
 template <class F>
 static int dispatch(const mf_plugin_params *tab, F &&f) {
@@ -1512,6 +1571,18 @@ int orc_box_cutcells(const mf_plugin_params *tab, const mf_params *prm, const mf
         p.T = &t;
         box_cutcells(p, B, sb, dx, prm->species_in_solid, prm->conj_ht);
         return MF_OK;
+    });
+}
+
+int orc_box_post(const mf_plugin_params *tab, const mf_params *prm, mf_box bx, mf_array4 state, double dt, int flags) {
+    Box B;
+    for (int d = 0; d < 3; ++d) { B.lo[d] = bx.lo[d]; B.hi[d] = bx.hi[d]; }
+    A4 sv = from_mf(state);
+    mf_plugin_params t = *tab;
+    return dispatch(&t, [&](auto p) {
+        p.T = &t;
+        return box_post(p, B, sv, dt, flags & MF_POST_CHEMISTRY, flags & MF_POST_PRIMITIVES, flags & MF_POST_CLIP,
+                        prm->conj_ht, prm->do_ns);
     });
 }
 

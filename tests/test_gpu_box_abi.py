@@ -170,3 +170,69 @@ def test_box_views_must_cover_the_box():
     assert st == abi.MF_ERR_BAD_ARG
     ctx.sync()
     ctx.close()
+
+
+@pytest.mark.parametrize("flags", [1, 2, 4, 7])
+def test_box_post(flags):
+    """mf_box_post (chemistry, primitive update, clip) against the oracle's
+    per-box form on a porous state with strong reactions, negative species and
+    perturbed conserved variables"""
+    import torch
+    from oracle.pyoracle import box_post
+    c = _case()
+    for r in range(2):
+        c.plugin.log_A[r] = np.log(3.0e10) + 0.5 * r
+        c.plugin.Ea[r] = 90.0e3
+    s = c.make_state()
+    r = np.random.default_rng(flags)
+    fl = s[abi.VFRAC] == 1.0
+    s[abi.RHO] *= 1.0 + 0.01 * r.standard_normal(fl.shape)
+    s[abi.RHOU] += 0.05 * s[abi.RHO] * r.standard_normal(fl.shape) * fl
+    s[abi.FLO_NVARS + 1] -= 0.02 * np.abs(s[abi.FLO_NVARS + 1]).max() * r.random(fl.shape)
+    lo, hi = (2, 1, 3), (15, 12, 9)
+    fab = np.ascontiguousarray(s[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1])
+    want = fab.copy()
+    st_o = box_post(c, (lo, hi), (want, lo), 2.0e-7, flags)
+    ctx = runtime.Context(c)
+    d = _dev(fab)
+    torch.cuda.synchronize()
+    st = runtime.lib().mf_box_post(ctx.h, abi.make_box(lo, hi), _view(d, lo, [hi[q] + 1 for q in range(3)], ctx.nv),
+                                   2.0e-7, flags)
+    assert st == abi.MF_OK
+    st_g = abi.MF_OK
+    try:
+        ctx.sync()
+    except abi.MFError as e:
+        st_g = e.status
+    got = d.cpu().numpy()
+    err = rel_err(want, got).max()
+    changed = np.any(want != fab)
+    say("box post flags %d: max rel err %.2e, oracle status %d, gpu %d, state changed %s" % (flags, err, st_o, st_g,
+                                                                                            changed))
+    assert st_o == st_g == abi.MF_OK and changed
+    assert err <= TOL
+    ctx.close()
+
+
+def test_box_post_negative_pressure():
+    from oracle.pyoracle import box_post
+    import torch
+    c = _case()
+    s = c.make_state()
+    solid = np.argwhere(s[abi.VFRAC] == 0.0)
+    s[abi.PRES][tuple(solid[len(solid) // 2])] = -1.0
+    fab = np.ascontiguousarray(s)
+    nx, ny, nz = c.n_cell
+    bx = ((0, 0, 0), (nx - 1, ny - 1, nz - 1))
+    want = fab.copy()
+    st_o = box_post(c, bx, (want, (0, 0, 0)), 1e-9, abi.MF_POST_PRIMITIVES)
+    ctx = runtime.Context(c)
+    d = _dev(fab)
+    torch.cuda.synchronize()
+    assert runtime.lib().mf_box_post(ctx.h, abi.make_box(*bx), _view(d, (0, 0, 0), (nx, ny, nz), ctx.nv), 1e-9,
+                                     abi.MF_POST_PRIMITIVES) == abi.MF_OK
+    with pytest.raises(abi.MFError) as e:
+        ctx.sync()
+    say("box post negative pressure: oracle %d gpu %d" % (st_o, e.value.status))
+    assert st_o == e.value.status == abi.MF_ERR_NEG_PRESSURE
+    ctx.close()
