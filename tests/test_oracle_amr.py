@@ -249,3 +249,31 @@ def test_tagging_ghost_layer_and_directions():
         sl[2 - d] = 0
         want[tuple(sl)] = 1
         assert np.array_equal(t, want), d
+
+
+def test_box_post_primitives_idempotent_and_clip_exact():
+    """orc_box_post: on a state whose primitives are consistent, the primitive
+    update changes nothing beyond rounding (a second pass nothing at all);
+    clipping zeroes exactly the negative species and touches nothing else"""
+    from mesoflow_amd import abi, cases
+    c = cases.porous_reactor(n_cell=(12, 10, 8), nspec=5, nreact=2, geometry="spheres", porosity=0.6, rmin=2.0,
+                             rmax=3.0, order_hyp=5, seed=3)
+    s = np.ascontiguousarray(c.make_state())
+    nx, ny, nz = c.n_cell
+    bx = ((0, 0, 0), (nx - 1, ny - 1, nz - 1))
+    a = s.copy()
+    assert po.box_post(c, bx, (a, (0, 0, 0)), 0.0, abi.MF_POST_PRIMITIVES) == 0
+    scale = np.abs(s).reshape(s.shape[0], -1).max(1)
+    assert np.all(np.abs(a - s).reshape(s.shape[0], -1).max(1) <= 1e-14 * np.maximum(scale, 1e-300))
+    b = a.copy()
+    po.box_post(c, bx, (b, (0, 0, 0)), 0.0, abi.MF_POST_PRIMITIVES)
+    assert np.array_equal(a, b)
+    rng = np.random.default_rng(1)
+    neg = rng.random(s[abi.FLO_NVARS:].shape) < 0.3
+    d = s.copy()
+    d[abi.FLO_NVARS:][neg] = -np.abs(d[abi.FLO_NVARS:][neg]) - 1e-3
+    e = d.copy()
+    po.box_post(c, bx, (e, (0, 0, 0)), 0.0, abi.MF_POST_CLIP)
+    assert np.all(e[abi.FLO_NVARS:][neg] == 0.0)
+    assert np.array_equal(e[abi.FLO_NVARS:][~neg], d[abi.FLO_NVARS:][~neg])
+    assert np.array_equal(e[:abi.FLO_NVARS], d[:abi.FLO_NVARS])
