@@ -17,7 +17,8 @@
 // IB least-squares normal and IDW weights against brute force; the masking
 // table of update_residual; SSP-RK3 on linear ODEs; the Sod exact solution of
 // tests/Cartesian/ShockTube_N2/exact_soln; direction-permutation and mirror
-// symmetry; conservation with periodic boundaries). The AMReX-dependent pieces
+// symmetry; conservation with periodic boundaries; the per-box post step:
+// idempotent primitive update, exact clip). The AMReX-dependent pieces
 // (FillPatchSingleLevel corner order, TimeIntegrator RK3 arithmetic, norm0 /
 // norm2 summation order) are "parity unpinned": AMReX is not in the tree.
 
