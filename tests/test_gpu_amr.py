@@ -229,3 +229,27 @@ def test_fill_patch_two_levels():
         say("fill patch two levels, box %d: bitwise %s" % (q, _same(got, w)))
         assert _same(got, w)
     fctx.close()
+
+
+def test_fill_patch_two_levels_rejects_a_physical_boundary():
+    """a fine box whose ghost region leaves the domain in a non-periodic
+    direction needs the fine level's boundary fill, which this call does not
+    apply: MF_ERR_UNSUPPORTED before any launch"""
+    from mesoflow_amd import abi
+    c = cases.shocktube_n2(n_cell=(32, 32, 32))   # x: ext_dir / wall, y and z periodic
+    fctx = runtime.Context(c)
+    nv = fctx.nv
+    rng = np.random.default_rng(2)
+    co = _dev(_rand(rng, (-3, -3, -3), (18, 18, 18), nv))
+    box = ((0, 8, 8), (7, 15, 15))
+    fab = _dev(_rand(rng, *box, nv))
+    sb = _dev(_rand(rng, (-3, 5, 5), (10, 18, 18), nv))
+    with pytest.raises(abi.MFError) as e:
+        amr.fill_patch_two_levels(fctx, box, sb, [(box, fab)], co, co, 0.0, 1.0, 0.5, 2)
+    assert e.value.status == abi.MF_ERR_UNSUPPORTED
+    inner = ((8, 0, 8), (15, 7, 15))   # touches y = 0: periodic, accepted
+    fab2 = _dev(_rand(rng, *inner, nv))
+    sb2 = _dev(_rand(rng, (5, -3, 5), (18, 10, 18), nv))
+    amr.fill_patch_two_levels(fctx, inner, sb2, [(inner, fab2)], co, co, 0.0, 1.0, 0.5, 2)
+    fctx.sync()
+    fctx.close()
