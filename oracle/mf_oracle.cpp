@@ -18,7 +18,15 @@
 // table of update_residual; SSP-RK3 on linear ODEs; the Sod exact solution of
 // tests/Cartesian/ShockTube_N2/exact_soln; direction-permutation and mirror
 // symmetry; conservation with periodic boundaries; the per-box post step:
-// idempotent primitive update, exact clip). The AMReX-dependent pieces
+// idempotent primitive update, exact clip; the viscous stress and work, heat
+// conduction and species diffusion against the exact eigenvalue of the
+// discrete second difference on 1-D Fourier modes and second-order
+// convergence to the continuous operators of a 3-D field; the LLF eigenvalue
+// against max|v_n| + max c and the plateau-step Lax-Friedrichs flux; subsonic
+// AUSM+-up against a transcription of Liou 2006). Reference quirks Q1 (mu and
+// k means with the x-neighbour), Q3 (presR with the left species, pinned to
+// the reference's text only) and Q4 (the weightR test) have no mathematical
+// counterpart. The AMReX-dependent pieces
 // (FillPatchSingleLevel corner order, TimeIntegrator RK3 arithmetic, norm0 /
 // norm2 summation order) are "parity unpinned": AMReX is not in the tree.
 
