@@ -137,6 +137,8 @@ typedef struct mf_plugin_params {
     double in_spec[MF_MAX_SPECIES];
     double out_pres;                     /* catalystCylinder outflow p0        */
     double siteconc;
+    double user[8];                      /* parameters of a case written against
+                                            the plugin API (its problem.* inputs) */
 } mf_plugin_params;
 
 /* ------------------------------------------------------------------------ */
@@ -151,6 +153,16 @@ int mf_device_count(int *count);
 int mf_ctx_create(const mf_plugin_params *plugin, const mf_params *params,
                   const mf_geom *geom, void **ctx_out);
 int mf_ctx_destroy(void *ctx);
+/* Plugin families compiled from case headers written against the plugin API
+ * (mesoflow_amd/csrc/mf_plugin_api.h; kinds >= 100): the table a context of
+ * `kind` starts from -- zeroed, kind, R_univ = RU, then the case's species
+ * tables (its species.cpp init(): nspec, ngas, molwt, advect_flag, ...);
+ * MF_ERR_UNSUPPORTED for a kind that is not such a family. The reference has
+ * these tables as compile-time globals (e.g.
+ * tests/Cartesian/SpeciesEqTests/species.cpp:10-24). Host only. */
+int mf_plugin_defaults(int kind, mf_plugin_params *tab);
+/* The case (namespace) name of such a family, a static string. */
+int mf_plugin_name(int kind, const char **name);
 /* Replaces the plugin parameter table of a context (same kind, nspec, ngas;
  * MF_ERR_BAD_ARG otherwise), e.g. new rate constants or inflow state. Work
  * enqueued afterwards uses the new table. No counterpart in the reference
@@ -229,11 +241,13 @@ int mf_box_interp_cons(void *ctx, mf_box fbx, mf_array4 crse, mf_array4 fine, in
  * cells under the region grown by one: ghost-filled coarse arrays) mapped by
  * mf_box_interp_cons, then the fine data of every fine box (fboxes/fdata, the
  * fine level's valid data at `time`, fbx's own included) and of its periodic
- * images where they cover the region. ctx is the fine level's context (its
- * geometry gives the domain and periodicity). A region that leaves the domain
- * in a non-periodic direction is MF_ERR_UNSUPPORTED (the physical boundary
- * fill of the fine level is not applied here). Allocates and frees a scratch
- * array for the coarse data and waits for the context's stream. */
+ * images where they cover the region, then, where the region leaves the
+ * domain in a non-periodic direction, the fine level's physical boundary
+ * conditions (the rules of mf_fill_ghosts_single_level: foextrap, wall
+ * reflection, the plugin's externalbc at `time`). ctx is the fine level's
+ * context (its geometry gives the domain, periodicity and boundary codes);
+ * fbx must lie inside the domain. The coarse data goes to a device buffer the
+ * context keeps and grows; the call is asynchronous on the context's stream. */
 int mf_fill_patch_two_levels(void *ctx, mf_box fbx, mf_array4 sborder, const mf_box *fboxes, const mf_array4 *fdata,
                              int nfine, mf_array4 crse_old, mf_array4 crse_new, double t_old, double t_new, double time,
                              int ratio);
@@ -314,10 +328,14 @@ int mf_level_evolve(void *level, double final_time, int max_step, int only_flow,
  * react_time while tchem < dt_react, then update_vars_after_chemsolve
  * (:74-140). The first stage of a substep reads the reference's other state
  * buffer (phi_old after its swap): the state before the previous flow step or
- * substep; after mf_level_set_state, the state itself. Returns
+ * substep; after mf_level_set_state, the state itself. With
+ * using_bg_inertgas = 1 the phase starts with store_inertgas_conc (:142-170:
+ * the background gas concentration into the density slot of every fluid cell)
+ * and update_vars_after_chemsolve forms the density with it (:94-97); only a
+ * family whose thermo has a background gas accepts that flag. Returns
  * MF_ERR_UNSUPPORTED after coupled steps (mf_level_step, the adjoint), whose
- * pre-chemistry state is not kept, and with using_bg_inertgas = 1 (no plugin
- * here has a background gas). All substeps of a cell run in one thread. */
+ * pre-chemistry state is not kept, and for using_bg_inertgas = 1 with a
+ * family without a background gas. All substeps of a cell run in one thread. */
 int mf_level_advance_chemistry(void *level, double react_time, double dt_react, double dt_react_rk);
 
 /* Evolve_split (Source/mflo_splitchem.cpp:24-72; main.cpp:36-56 reads its
@@ -343,14 +361,16 @@ int mf_level_tag(void *level, double time, const double *refine_phi, const doubl
  * Source/mflo_evolve.cpp:177-209 -> MakeNewGrids with amr.n_error_buf,
  * blocking_factor, max_grid_size, grid_eff and the refinement ratio). tags:
  * host bytes over the domain n_cell (x fastest); every tag grows by
- * n_error_buf, tags are coarsened by blocking_factor/ratio, clustered
+ * n_error_buf (wrapping around the domain where is_periodic[d] is set;
+ * is_periodic may be NULL: no periodic direction), tags are coarsened by blocking_factor/ratio, clustered
  * (Berger-Rigoutsos: accept at tags/volume >= grid_eff, else cut at a hole,
  * an inflection of the signature, or in half), refined to the fine level and
  * chopped to max_grid_size. Boxes are in fine-level indices, disjoint,
  * aligned to blocking_factor, and cover every buffered tag. With boxes NULL
  * and cap 0 only *nboxes is returned. Host code: needs no device. */
-int mf_tags_to_boxes(const unsigned char *tags, const int32_t n_cell[3], int n_error_buf, int blocking_factor,
-                     int max_grid_size, double grid_eff, int ratio, mf_box *boxes, int cap, int *nboxes);
+int mf_tags_to_boxes(const unsigned char *tags, const int32_t n_cell[3], const int32_t *is_periodic, int n_error_buf,
+                     int blocking_factor, int max_grid_size, double grid_eff, int ratio, mf_box *boxes, int cap,
+                     int *nboxes);
 
 /* Box list, geometry and component count of a level (boxes may be NULL to
  * query the count; cap is the capacity of boxes). */

@@ -37,6 +37,10 @@ BC_PERIODIC, BC_FOEXTRAP, BC_EXT_DIR, BC_WALL = 0, 2, 3, 6
 PLUGIN_SHOCKTUBE_N2 = 1
 PLUGIN_CATALYST_CYLINDER = 2
 PLUGIN_POROUS_ARRHENIUS = 3
+# case families compiled from mesoflow_amd/plugins/<case>/mflo_case.H (their PLUGIN_ID)
+PLUGIN_SPECIES_EQ_TESTS = 101
+PLUGIN_CHANNEL = 102
+PLUGIN_BG_INERT_GAS = 103
 
 MODE_PER_BOX, MODE_BATCHED, MODE_GRAPH = 0, 1, 2
 MF_POST_CHEMISTRY, MF_POST_PRIMITIVES, MF_POST_CLIP = 1, 2, 4
@@ -80,7 +84,7 @@ class PluginParams(C.Structure):
                 ("right_p", C.c_double), ("right_rho", C.c_double), ("right_T", C.c_double),
                 ("left_spec", _S), ("right_spec", _S),
                 ("in_vel", C.c_double), ("in_temp", C.c_double), ("in_rho", C.c_double), ("in_pres", C.c_double),
-                ("in_spec", _S), ("out_pres", C.c_double), ("siteconc", C.c_double)]
+                ("in_spec", _S), ("out_pres", C.c_double), ("siteconc", C.c_double), ("user", C.c_double * 8)]
 
 
 def make_box(lo, hi):

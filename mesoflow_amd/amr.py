@@ -88,13 +88,15 @@ def level_tag(level, time, refine_phi, refine_phigrad, comps):
     return tags, cnt.value
 
 
-def tags_to_boxes(tags, n_error_buf=1, blocking_factor=8, max_grid_size=32, grid_eff=0.7, ratio=2):
-    """new fine-level boxes [(lo, hi)] from host tags [nz][ny][nx] (mf_tags_to_boxes)"""
+def tags_to_boxes(tags, n_error_buf=1, blocking_factor=8, max_grid_size=32, grid_eff=0.7, ratio=2, periodic=None):
+    """new fine-level boxes [(lo, hi)] from host tags [nz][ny][nx] (mf_tags_to_boxes);
+    periodic: (px, py, pz) flags of the directions where buffered tags wrap"""
     t = np.ascontiguousarray(tags, dtype=np.uint8)
     nz, ny, nx = t.shape
     n = (C.c_int32 * 3)(nx, ny, nz)
+    per = None if periodic is None else (C.c_int32 * 3)(*[int(bool(v)) for v in periodic])
     nb = C.c_int()
-    args = (n, int(n_error_buf), int(blocking_factor), int(max_grid_size), float(grid_eff), int(ratio))
+    args = (n, per, int(n_error_buf), int(blocking_factor), int(max_grid_size), float(grid_eff), int(ratio))
     lib = runtime.lib()
     runtime.check(lib.mf_tags_to_boxes(t.ctypes.data_as(C.c_void_p), *args, None, 0, C.byref(nb)), "mf_tags_to_boxes")
     out = (abi.Box * max(nb.value, 1))()

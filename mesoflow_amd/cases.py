@@ -441,3 +441,149 @@ def porous_reactor(n_cell=(256, 256, 256), nspec=5, nreact=2, geometry="spheres"
 
     bl = boxes if boxes is not None else (chop_boxes(n_cell, max_grid_size) if max_grid_size else [])
     return Case("porous_reactor", t, p, g, make_state, bl, nsteps)
+
+
+# ---------------------------------------------------------------------------
+# Cases written against the plugin API (mesoflow_amd/plugins/<case>/mflo_case.H);
+# their species tables come from the library (mf_plugin_defaults)
+# ---------------------------------------------------------------------------
+def _case_plugin(kind):
+    from . import runtime
+    return runtime.plugin_defaults(kind)
+
+
+def species_eq_tests(n_cell=(32, 4, 4), direction=0, cfl=0.8, nsteps=7000, max_grid_size=None):
+    """tests/Cartesian/SpeciesEqTests (inputs_x/y/z): a unit-length channel
+    along `direction` (0.125 across), ext_dir at both of its ends, walls across,
+    no flow solve (mflo.solve_navier_stokes = 0); Prob.H:13-81: p = rho = 1,
+    unit velocity along the channel, T from the thermo, no species."""
+    t = _case_plugin(abi.PLUGIN_SPECIES_EQ_TESTS)
+    lo_bc = [6, 6, 6]
+    hi_bc = [6, 6, 6]
+    lo_bc[direction] = hi_bc[direction] = abi.BC_EXT_DIR
+    p = default_params(order_hyp=2, cfl=cfl, lo_bc=lo_bc, hi_bc=hi_bc)
+    p.do_ns = 0
+    p.stop_time = 1.0
+    hi = [0.125, 0.125, 0.125]
+    hi[direction] = 1.0
+    g = make_geom((0.0, 0.0, 0.0), tuple(hi), n_cell, (0, 0, 0))
+    nx, ny, nz = n_cell
+
+    def make_state():
+        nv = abi.FLO_NVARS + 2
+        s = np.zeros((nv, nz, ny, nx))
+        gamma = 1.4
+        s[abi.PRES] = 1.0
+        s[abi.DENS] = 1.0
+        s[abi.TEMP] = 1.0                              # p / rho / R, R = 1
+        s[abi.VELX + direction] = 1.0
+        s[abi.RHO] = 1.0
+        s[abi.RHOU + direction] = 1.0
+        s[abi.RHOE] = 1.0 / (gamma - 1.0) + 0.5
+        s[abi.VFRAC] = 1.0
+        return s
+
+    boxes = chop_boxes(n_cell, max_grid_size) if max_grid_size else []
+    return Case("species_eq_tests", t, p, g, make_state, boxes, nsteps)
+
+
+def channel(n_cell=(128, 32, 32), direction=0, cfl=0.1, order_hyp=2, nsteps=4000, max_grid_size=None):
+    """tests/Cartesian/Channel (inputs_x/y/z): periodic along `direction`,
+    no-slip walls across (0.125 wide); Prob.H: p = rho = 1, axial velocity 0.1,
+    concentration 1, T from the thermo."""
+    t = _case_plugin(abi.PLUGIN_CHANNEL)
+    lo_bc = [abi.BC_WALL] * 3
+    hi_bc = [abi.BC_WALL] * 3
+    lo_bc[direction] = hi_bc[direction] = abi.BC_PERIODIC
+    p = default_params(order_hyp=order_hyp, cfl=cfl, lo_bc=lo_bc, hi_bc=hi_bc)
+    p.stop_time = 0.1
+    hi = [0.125, 0.125, 0.125]
+    hi[direction] = 1.0
+    per = [0, 0, 0]
+    per[direction] = 1
+    g = make_geom((0.0, 0.0, 0.0), tuple(hi), n_cell, per)
+    nx, ny, nz = n_cell
+    R_air = RU / t.molwt[0]
+
+    def make_state():
+        nv = abi.FLO_NVARS + 1
+        s = np.zeros((nv, nz, ny, nx))
+        gamma = 1.4
+        u = 0.1
+        s[abi.PRES] = 1.0
+        s[abi.DENS] = 1.0
+        s[abi.TEMP] = 1.0 / R_air / 1.0
+        s[abi.VELX + direction] = u
+        s[abi.RHO] = 1.0
+        s[abi.RHOU + direction] = u
+        s[abi.RHOE] = 1.0 / (gamma - 1.0) + 0.5 * u * u
+        s[abi.VFRAC] = 1.0
+        s[abi.FLO_NVARS] = 1.0
+        return s
+
+    boxes = chop_boxes(n_cell, max_grid_size) if max_grid_size else []
+    return Case("channel", t, p, g, make_state, boxes, nsteps)
+
+
+def _bg_thermo(t, rho, pres, spec):
+    """thermo.H of py_utils/rxnscheme_writer/example (hydrogen + background gas):
+    temperature and specific internal energy from rho, p and the species."""
+    M_bg, g_bg = 0.04, 1.67
+    sum_ciMi = spec[0] * t.molwt[0]
+    c_bg = (rho - sum_ciMi) / M_bg
+    sum_ci = spec[0] + c_bg
+    temp = pres / (sum_ci * RU)
+    cv = RU / (t.gamma[0] - 1.0) * spec[0] + RU / (g_bg - 1.0) * c_bg
+    return temp, cv * temp / rho
+
+
+def bg_inert_gas(n_cell=(48, 24, 24), porosity=0.6, rmin=3.0, rmax=5.0, seed=20261209, dx=4.0e-6, order_hyp=5,
+                 cfl=0.3, do_ns=1, fs_vel=1.0, fs_pres=1.0e5, fs_temp=700.0, fs_h2=10.0, sites=10.0, h2_initial=0.0,
+                 nsteps=20, max_grid_size=None):
+    """py_utils/rxnscheme_writer/example (inputs, userfuncs.cpp, Prob.H): hydrogen
+    (free stream 10 mol/m^3 at 700 K, 1e5 Pa, 1 m/s) in a background inert gas
+    entering at x-lo (ext_dir), foextrap elsewhere, with chemistry on; the
+    tomography solid of the example (a data file the reference does not ship)
+    is replaced by a seeded sphere pack (corner-sampled volume fractions);
+    catalyst sites in the solid. The gas starts at the free-stream pressure and
+    temperature with h2_initial mol/m^3 of hydrogen in the background gas (the
+    example's Prob.H pairs the free-stream density, hydrogen included, with
+    zero concentrations, i.e. a background gas at about 1570 K beside 700 K
+    in the solid). The example's rates (1.7e11 and 1.7e10 m^3/mol/s) with 10
+    mol/m^3 of sites are far too stiff for explicit steps of the flow's dt;
+    coupled runs use a small `sites`."""
+    t = _case_plugin(abi.PLUGIN_BG_INERT_GAS)
+    t.in_vel, t.in_pres, t.in_temp = fs_vel, fs_pres, fs_temp
+    t.in_spec[0] = fs_h2
+    t.user[0] = 1.0                                   # problem.chemistry_on
+    p = default_params(order_hyp=order_hyp, cfl=cfl, lo_bc=(3, 2, 2), hi_bc=(2, 2, 2))
+    p.do_ns = do_ns
+    p.using_bg_inertgas = 1
+    prob_hi = tuple(dx * n for n in n_cell)
+    g = make_geom((0.0, 0.0, 0.0), prob_hi, n_cell, (0, 0, 0))
+    nx, ny, nz = n_cell
+
+    def make_state():
+        vf = vfrac_from_corners(sphere_pack_solid(n_cell, porosity, rmin, rmax, seed))
+        nv = abi.FLO_NVARS + 3
+        s = np.zeros((nv, nz, ny, nx))
+        spec = np.array([h2_initial, 0.0, 0.0])
+        ctot = fs_pres / RU / fs_temp
+        rho0 = h2_initial * t.molwt[0] + (ctot - h2_initial) * 0.04      # get_r_from_tpc
+        fluid = vf >= 1.0
+        u = np.where(fluid, fs_vel, 0.0)
+        _, e0 = _bg_thermo(t, rho0, fs_pres, spec)
+        s[abi.PRES] = fs_pres
+        s[abi.DENS] = rho0
+        s[abi.VELX] = u
+        s[abi.TEMP] = fs_temp
+        s[abi.VFRAC] = vf
+        s[abi.FLO_NVARS + 0] = h2_initial
+        s[abi.FLO_NVARS + 2] = (1.0 - vf) * sites
+        s[abi.RHO] = rho0
+        s[abi.RHOU] = rho0 * u
+        s[abi.RHOE] = rho0 * e0 + 0.5 * rho0 * u * u
+        return s
+
+    boxes = chop_boxes(n_cell, max_grid_size) if max_grid_size else []
+    return Case("bg_inert_gas", t, p, g, make_state, boxes, nsteps)

@@ -24,6 +24,8 @@ using namespace mfd;
 
 int mf_ctx_stream(void *ctx, hipStream_t *stream, int *nv);  // mf_capi.hip
 int mf_ctx_geom(void *ctx, mf_geom *geom);                   // mf_capi.hip
+int mf_ctx_aux(void *ctx, size_t bytes, double **p);         // mf_capi.hip
+int mf_ctx_fill_physbc(void *ctx, mf_array4 view, mf_box region, double time);  // mf_capi.hip
 
 namespace {
 
@@ -243,10 +245,13 @@ __global__ __launch_bounds__(kBlock) void k_tag(Ext b, View s, unsigned char *ta
         for (int c = 0; c < tv.n; ++c) {
             const int n = tv.comp[c];
             const double u = s(i, j, k, n);
-            const double gx = fmax(fabs(s(i + 1, j, k, n) - u), fabs(u - s(i - 1, j, k, n)));
-            const double gy = fmax(fabs(s(i, j + 1, k, n) - u), fabs(u - s(i, j - 1, k, n)));
-            const double gz = fmax(fabs(s(i, j, k + 1, n) - u), fabs(u - s(i, j, k - 1, n)));
-            if (fmax(gx, fmax(gy, gz)) > tv.grad[c]) set = true;
+            // amrex::max is std::max ((a < b) ? b : a: a NaN first argument
+            // propagates, a NaN second argument is dropped), and
+            // amrex::max(gx, gy, gz) = max(max(gx, gy), gz)
+            const double gx = std_max(fabs(s(i + 1, j, k, n) - u), fabs(u - s(i - 1, j, k, n)));
+            const double gy = std_max(fabs(s(i, j + 1, k, n) - u), fabs(u - s(i, j - 1, k, n)));
+            const double gz = std_max(fabs(s(i, j, k + 1, n) - u), fabs(u - s(i, j, k - 1, n)));
+            if (std_max(std_max(gx, gy), gz) > tv.grad[c]) set = true;
         }
         if (set) tags[q] = tagval;
     }
@@ -286,7 +291,7 @@ int mf_fill_patch_two_levels(void *ctx, mf_box fbx, mf_array4 sborder, const mf_
     for (int d = 0; d < 3; ++d) {
         gb.lo[d] = fbx.lo[d] - MF_NGROW;
         gb.hi[d] = fbx.hi[d] + MF_NGROW;
-        if (!g.is_periodic[d] && (gb.lo[d] < 0 || gb.hi[d] >= g.n_cell[d])) return MF_ERR_UNSUPPORTED;
+        if (fbx.lo[d] < 0 || fbx.hi[d] >= g.n_cell[d]) return MF_ERR_BAD_ARG;
         if (g.n_cell[d] % ratio) return MF_ERR_BAD_ARG;
     }
     if (!holds(sborder, gb.lo, gb.hi, nv)) return MF_ERR_BAD_ARG;
@@ -301,7 +306,8 @@ int mf_fill_patch_two_levels(void *ctx, mf_box fbx, mf_array4 sborder, const mf_
     if (!holds(crse_old, cb.lo, cb.hi, nv) || !holds(crse_new, cb.lo, cb.hi, nv)) return MF_ERR_BAD_ARG;
     const Ext ce = ext_of(cb);
     double *tmp = nullptr;
-    if (hipMalloc(&tmp, (size_t)ce.cells * nv * sizeof(double)) != hipSuccess) return MF_ERR_HIP;
+    rc = mf_ctx_aux(ctx, (size_t)ce.cells * nv * sizeof(double), &tmp);
+    if (rc) return rc;
     mf_array4 ct{};
     ct.p = tmp;
     ct.jstride = ce.e[0];
@@ -337,8 +343,9 @@ int mf_fill_patch_two_levels(void *ctx, mf_box fbx, mf_array4 sborder, const mf_
                     if (hipGetLastError() != hipSuccess) rc = MF_ERR_HIP;
                 }
     }
-    hipStreamSynchronize(st);
-    hipFree(tmp);
+    // 4. the fine level's physical boundary conditions where the region leaves
+    // the domain in a non-periodic direction
+    if (!rc) rc = mf_ctx_fill_physbc(ctx, sborder, gb, time);
     return rc;
 }
 

@@ -176,6 +176,8 @@ struct ChemSplitArgs {
     int nsub;                  // substeps of this launch
     double dt;                 // react_time_step
     int finish;                // 1: update_vars_after_chemsolve after the substeps
+    int bg;                    // mflo.using_bg_inertgas (a family with a background gas)
+    int store_bg;              // 1: first launch of a phase: store_inertgas_conc first
     int do_ns;
     int *err;
 };
@@ -193,7 +195,10 @@ struct BoxPostArgs {
 // Host launchers of one compiled plugin family.
 struct KernelSet {
     int ns;
+    bool has_bg_gas;           // the family's thermo has a background gas (using_bg_inertgas)
     void (*fill)(const FillArgs &, int phase, hipStream_t);
+    // physical-boundary phase of the fill over the region [rlo, rhi] of a.s
+    void (*fill_region)(const FillArgs &, int phase, const int *rlo, const int *rhi, hipStream_t);
     void (*cut)(const CutArgs &, hipStream_t);
     void (*cut_geom)(const CutArgs &, double *geom, unsigned *mask, hipStream_t);
     void (*cut_box)(const CutArgs &, hipStream_t);
@@ -208,6 +213,19 @@ struct KernelSet {
     void (*box_post)(const BoxPostArgs &, hipStream_t);
     const AdjSet *adj;         // discrete adjoint (null: not compiled for this family)
 };
+
+// A plugin family compiled from a case header written against
+// mf_plugin_api.h (mf_kernels_case.hip), registered at library load.
+struct CaseFamily {
+    int id;                                  // mf_plugin_params::kind
+    int nspec;
+    bool has_bg_gas;
+    const KernelSet *ks;
+    void (*init_species)(mf_plugin_params *);
+    const char *name;
+};
+int register_case_family(const CaseFamily &f);   // mf_capi.hip
+const CaseFamily *find_case_family(int id);
 
 extern const KernelSet ks_shocktube_n2;
 extern const KernelSet ks_catalyst_cylinder;

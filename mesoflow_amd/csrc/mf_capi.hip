@@ -20,6 +20,23 @@
 
 using namespace mfd;
 
+namespace mfd {
+// the families compiled from case headers (filled while the library loads)
+static std::vector<CaseFamily> &case_families() {
+    static std::vector<CaseFamily> v;
+    return v;
+}
+int register_case_family(const CaseFamily &f) {
+    case_families().push_back(f);
+    return (int)case_families().size();
+}
+const CaseFamily *find_case_family(int id) {
+    for (const CaseFamily &f : case_families())
+        if (f.id == id) return &f;
+    return nullptr;
+}
+}  // namespace mfd
+
 namespace {
 
 #define HIPCHK(x)                                      \
@@ -32,6 +49,7 @@ namespace {
     } while (0)
 
 thread_local hipError_t last_hip_error = hipSuccess;
+
 
 struct Ctx {
     mf_plugin_params tab;
@@ -47,7 +65,12 @@ struct Ctx {
     // scratch of the per-box C-ABI calls
     double *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // work buffer of the two-level operators (coarse data of a fill)
+    double *aux = nullptr;
+    size_t aux_bytes = 0;
     int tsh[3] = {32, 4, 2};
+    // tiles of the LDS-staged WENO5 face sweeps of the per-box calls (x, y, z)
+    int ftsh[3][3] = {{64, 4, 1}, {32, 8, 1}, {32, 1, 8}};
 };
 
 // HIP-event timing of the kernel classes of a step (mf_level_step_timed)
@@ -134,6 +157,7 @@ struct AdjState {
 };
 
 const KernelSet *pick_kernels(const mf_plugin_params &t) {
+    if (const CaseFamily *f = find_case_family(t.kind)) return t.nspec == f->nspec ? f->ks : nullptr;
     switch (t.kind) {
     case MF_PLUGIN_SHOCKTUBE_N2: return (t.nspec == 1 && t.ngas == 1) ? &ks_shocktube_n2 : nullptr;
     case MF_PLUGIN_CATALYST_CYLINDER: return (t.nspec == 4 && t.ngas == 3) ? &ks_catalyst_cylinder : nullptr;
@@ -1037,16 +1061,29 @@ int read_ts(Level *L, TimeState *h) {
     return MF_OK;
 }
 
+// grows the context's scratch buffer; after a failed allocation the context
+// holds no buffer (scratch null, scratch_bytes 0), so a later smaller request
+// allocates again instead of using a freed pointer
 int ensure_scratch(Ctx *c, size_t bytes) {
-    if (bytes <= c->scratch_bytes) return MF_OK;
-    if (c->scratch) { HIPCHK(hipStreamSynchronize(c->stream)); hipFree(c->scratch); c->scratch = nullptr; }
-    HIPCHK(hipMalloc(&c->scratch, bytes));
+    if (c->scratch && bytes <= c->scratch_bytes) return MF_OK;
+    if (c->scratch) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        hipFree(c->scratch);
+        c->scratch = nullptr;
+    }
+    c->scratch_bytes = 0;
+    double *p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return MF_ERR_HIP;
+    }
+    c->scratch = p;
     c->scratch_bytes = bytes;
     return MF_OK;
 }
 
-int upload_tab(Ctx *c, const mf_plugin_params *plugin) {
-    Tab ht;
+// the device table: the C-ABI table, per-species constants and the geometry
+void make_tab(const mf_plugin_params *plugin, const mf_geom &g, Tab &ht) {
     std::memset(&ht, 0, sizeof(ht));
     std::memcpy(static_cast<mf_plugin_params *>(&ht), plugin, sizeof(mf_plugin_params));
     for (int q = 0; q < MF_MAX_SPECIES; ++q) {
@@ -1055,6 +1092,16 @@ int upload_tab(Ctx *c, const mf_plugin_params *plugin) {
         ht.cv[q] = plugin->R_univ / (plugin->gamma[q] - 1.0);
         ht.cvm[q] = plugin->R_univ / (plugin->gamma[q] - 1.0) / plugin->molwt[q];
     }
+    for (int d = 0; d < 3; ++d) {
+        ht.geo.prob_lo[d] = g.prob_lo[d];
+        ht.geo.prob_hi[d] = g.prob_hi[d];
+        ht.geo.dx[d] = (g.prob_hi[d] - g.prob_lo[d]) / g.n_cell[d];
+    }
+}
+
+int upload_tab(Ctx *c, const mf_plugin_params *plugin) {
+    Tab ht;
+    make_tab(plugin, c->geom, ht);
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(c->d_tab, &ht, sizeof(Tab), hipMemcpyHostToDevice));
     c->tab = *plugin;
@@ -1119,14 +1166,7 @@ int mf_ctx_create(const mf_plugin_params *plugin, const mf_params *params, const
     for (int d = 0; d < 3; ++d) c->dx[d] = (geom->prob_hi[d] - geom->prob_lo[d]) / geom->n_cell[d];
     hipGetDevice(&c->device);
     Tab ht;
-    std::memset(&ht, 0, sizeof(ht));
-    std::memcpy(static_cast<mf_plugin_params *>(&ht), plugin, sizeof(mf_plugin_params));
-    for (int q = 0; q < MF_MAX_SPECIES; ++q) {
-        ht.gm1[q] = plugin->gamma[q] - 1.0;
-        ht.igm1[q] = 1.0 / (plugin->gamma[q] - 1.0);
-        ht.cv[q] = plugin->R_univ / (plugin->gamma[q] - 1.0);
-        ht.cvm[q] = plugin->R_univ / (plugin->gamma[q] - 1.0) / plugin->molwt[q];
-    }
+    make_tab(plugin, *geom, ht);
     if (hipMalloc(&c->d_tab, sizeof(Tab)) != hipSuccess || hipMalloc(&c->d_err, sizeof(int)) != hipSuccess ||
         hipMemcpy(c->d_tab, &ht, sizeof(Tab), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(c->d_err, 0, sizeof(int)) != hipSuccess) {
@@ -1137,6 +1177,25 @@ int mf_ctx_create(const mf_plugin_params *plugin, const mf_params *params, const
     return MF_OK;
 }
 
+int mf_plugin_defaults(int kind, mf_plugin_params *tab) {
+    if (!tab) return MF_ERR_BAD_ARG;
+    const CaseFamily *f = find_case_family(kind);
+    if (!f) return MF_ERR_UNSUPPORTED;
+    std::memset(tab, 0, sizeof(*tab));
+    tab->kind = kind;
+    tab->R_univ = mfp::RU;
+    f->init_species(tab);
+    return MF_OK;
+}
+
+int mf_plugin_name(int kind, const char **name) {
+    if (!name) return MF_ERR_BAD_ARG;
+    const CaseFamily *f = find_case_family(kind);
+    if (!f) return MF_ERR_UNSUPPORTED;
+    *name = f->name;
+    return MF_OK;
+}
+
 int mf_ctx_destroy(void *ctx) {
     Ctx *c = (Ctx *)ctx;
     if (!c) return MF_ERR_BAD_ARG;
@@ -1144,6 +1203,7 @@ int mf_ctx_destroy(void *ctx) {
     if (c->d_tab) hipFree(c->d_tab);
     if (c->d_err) hipFree(c->d_err);
     if (c->scratch) hipFree(c->scratch);
+    if (c->aux) hipFree(c->aux);
     delete c;
     return MF_OK;
 }
@@ -1208,8 +1268,10 @@ int mf_box_cutcells(void *ctx, mf_box bx, mf_array4 sborder, double time) {
     return MF_OK;
 }
 
-int mf_box_rhs(void *ctx, mf_box bx, mf_array4 sborder, mf_array4 dsdt, const mf_array4 *flux3, double time) {
-    Ctx *c = (Ctx *)ctx;
+// mf_box_rhs with the face fluxes either in flux3 or, with flux3 null, in
+// fluxbuf (cap doubles; null: the context's scratch buffer, allocated here)
+int box_rhs_impl(Ctx *c, mf_box bx, mf_array4 sborder, mf_array4 dsdt, const mf_array4 *flux3, double time,
+                 double *fluxbuf, size_t cap) {
     if (!c || !sborder.p || !dsdt.p || sborder.ncomp < c->nv || dsdt.ncomp < c->nv) return MF_ERR_BAD_ARG;
     int len[3];
     for (int d = 0; d < 3; ++d) {
@@ -1245,14 +1307,20 @@ int mf_box_rhs(void *ctx, mf_box bx, mf_array4 sborder, mf_array4 dsdt, const mf
             fl[d] += 1;
             tot += (size_t)fl[0] * fl[1] * fl[2] * NC;
         }
-        int st = ensure_scratch(c, tot * sizeof(double));
-        if (st) return st;
+        double *base = fluxbuf;
+        if (base) {
+            if (tot > cap) return MF_ERR_TOO_LARGE;
+        } else {
+            int st = ensure_scratch(c, tot * sizeof(double));
+            if (st) return st;
+            base = c->scratch;
+        }
         size_t ofs = 0;
         for (int d = 0; d < 3; ++d) {
             int fl[3] = {len[0], len[1], len[2]};
             fl[d] += 1;
             View &v = w.one.flux[d];
-            v.p = c->scratch + ofs;
+            v.p = base + ofs;
             v.b0 = bx.lo[0]; v.b1 = bx.lo[1]; v.b2 = bx.lo[2];
             v.js = fl[0];
             v.ks = (int64_t)fl[0] * fl[1];
@@ -1276,7 +1344,25 @@ int mf_box_rhs(void *ctx, mf_box bx, mf_array4 sborder, mf_array4 dsdt, const mf
     a.conj_ht = c->prm.conj_ht;
     a.flux_full = full;
     a.err = c->d_err;
-    for (int d = 0; d < 3; ++d) c->ks->faces(a, w, d, w.t1, 256, c->stream);
+    // WENO5 with the flow solve: the LDS-staged sweeps over face tiles of the
+    // box (the level path's kernels); the other schemes: one face per thread
+    const bool lds = a.order == 5 && a.nsflag;
+    for (int d = 0; d < 3; ++d) {
+        if (!lds) {
+            c->ks->faces(a, w, d, w.t1, 256, c->stream);
+            continue;
+        }
+        Work wf = w;
+        wf.cls = d;
+        for (int q = 0; q < 3; ++q) {
+            wf.tsh[q] = c->ftsh[d][q];
+            wf.one.ntile[d][q] = div_up(len[q], wf.tsh[q]);
+        }
+        wf.t0 = 0;
+        wf.t1 = box_tiles(wf.one, d);
+        c->ks->faces_lds(a, wf, d, std::min(wf.t1, 2048), face_block(c->ftsh[d]), c->ks->lds_bytes(c->ftsh[d], d),
+                         c->stream);
+    }
     ResArgs r;
     std::memset(&r, 0, sizeof(r));
     r.st = a;
@@ -1285,6 +1371,10 @@ int mf_box_rhs(void *ctx, mf_box bx, mf_array4 sborder, mf_array4 dsdt, const mf
     c->ks->residual(r, w, w.t1, 256, c->stream);
     HIPCHK(hipGetLastError());
     return MF_OK;
+}
+
+int mf_box_rhs(void *ctx, mf_box bx, mf_array4 sborder, mf_array4 dsdt, const mf_array4 *flux3, double time) {
+    return box_rhs_impl((Ctx *)ctx, bx, sborder, dsdt, flux3, time, nullptr, 0);
 }
 
 int mf_box_rk3_combine(void *ctx, int stage, mf_box bx, mf_array4 s_new, mf_array4 src, mf_array4 dsdt, double dt) {
@@ -1348,6 +1438,20 @@ int mf_level_create(void *ctx, const mf_box *boxes, int nboxes, void **level_out
             L->boxes.push_back(boxes[b]);
         }
         if (cov != ncell) { delete L; return MF_ERR_BAD_ARG; }  // boxes must tile the domain
+        if (nboxes > 1) {
+            // disjoint (with the volume check: they cover the domain); overlapping
+            // boxes would have tiles writing the same cells and leave a gap unupdated
+            std::vector<unsigned char> mark((size_t)ncell, 0);
+            for (int b = 0; b < nboxes; ++b)
+                for (int k = boxes[b].lo[2]; k <= boxes[b].hi[2]; ++k)
+                    for (int j = boxes[b].lo[1]; j <= boxes[b].hi[1]; ++j) {
+                        unsigned char *row = mark.data() + (size_t)L->n[0] * (j + (size_t)L->n[1] * k);
+                        for (int i = boxes[b].lo[0]; i <= boxes[b].hi[0]; ++i) {
+                            if (row[i]) { delete L; return MF_ERR_BAD_ARG; }
+                            row[i] = 1;
+                        }
+                    }
+        }
     }
     const int NC = NCV + c->ns;
     L->padded = (size_t)(L->n[0] + 6) * (L->n[1] + 6) * (L->n[2] + 6) * c->nv;
@@ -1597,7 +1701,7 @@ int level_evolve(Level *L, double final_time, int max_step, int only_flow, int m
 int level_chem_phase(Level *L, double react_time, double dt_react, double dt_rk) {
     (void)react_time;
     Ctx *c = L->c;
-    if (c->prm.using_bg_inertgas) return MF_ERR_UNSUPPORTED;
+    if (c->prm.using_bg_inertgas && !c->ks->has_bg_gas) return MF_ERR_UNSUPPORTED;
     if (!(dt_rk > 0.0) || !(dt_react < 1e300)) return MF_ERR_BAD_ARG;
     if (L->lag_mode == 0) return MF_ERR_UNSUPPORTED;
     int st = lag_alloc(L);
@@ -1620,12 +1724,14 @@ int level_chem_phase(Level *L, double react_time, double dt_react, double dt_rk)
     a.dt = dt_rk;
     a.do_ns = c->prm.do_ns;
     a.err = c->d_err;
+    a.bg = c->prm.using_bg_inertgas ? 1 : 0;
     // launches of at most kChunk substeps (the kernel continues the sequence exactly)
     const int64_t kChunk = 1024;
     int64_t done = 0;
     do {
         const int64_t m = std::min(kChunk, nsub - done);
         a.nsub = (int)m;
+        a.store_bg = done == 0;
         done += m;
         a.finish = done == nsub;
         c->ks->chem_split(a, c->stream);
@@ -1687,7 +1793,7 @@ int mf_level_evolve_split(void *level, double t_ss, double t_react, double dt_re
                           int max_step, int mode) {
     Level *L = (Level *)level;
     if (!L || mode < 0 || mode > 2 || max_step < 0) return MF_ERR_BAD_ARG;
-    if (L->c->prm.using_bg_inertgas) return MF_ERR_UNSUPPORTED;
+    if (L->c->prm.using_bg_inertgas && !L->c->ks->has_bg_gas) return MF_ERR_UNSUPPORTED;
     double react_time = 0.0, flow_time;
     int st = level_evolve(L, t_ss, max_step, 1, mode, nullptr);
     if (st) return st;
@@ -2038,5 +2144,65 @@ int mf_ctx_geom(void *ctx, mf_geom *geom) {
     Ctx *c = (Ctx *)ctx;
     if (!c || !geom) return MF_ERR_BAD_ARG;
     *geom = c->geom;
+    return MF_OK;
+}
+
+// a device buffer of at least `bytes` owned by the context, for mf_amr.hip:
+// grown when needed (after the stream's work that may use the old one),
+// reused by later calls in stream order
+int mf_ctx_aux(void *ctx, size_t bytes, double **p) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c || !p) return MF_ERR_BAD_ARG;
+    if (!c->aux || bytes > c->aux_bytes) {
+        if (c->aux) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            hipFree(c->aux);
+            c->aux = nullptr;
+        }
+        c->aux_bytes = 0;
+        double *q = nullptr;
+        if (hipMalloc(&q, bytes) != hipSuccess || !q) {
+            (void)hipGetLastError();
+            return MF_ERR_HIP;
+        }
+        c->aux = q;
+        c->aux_bytes = bytes;
+    }
+    *p = c->aux;
+    return MF_OK;
+}
+
+// the physical-boundary rules of the single-level fill (foextrap, wall
+// reflection, the plugin's externalbc; Source/mflo_bcfill.H:16-128) on the
+// cells of `region` of `view` outside the domain in non-periodic directions,
+// in phases of 1, 2, 3 such directions; every other cell of the region must
+// be filled already. For mf_amr.hip (FillPatchTwoLevels of a fine box).
+int mf_ctx_fill_physbc(void *ctx, mf_array4 view, mf_box region, double time) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c || !view.p || view.ncomp < c->nv) return MF_ERR_BAD_ARG;
+    bool outside = false;
+    for (int d = 0; d < 3; ++d) {
+        if (region.lo[d] < view.begin[d] || region.hi[d] >= view.end[d] || region.hi[d] < region.lo[d])
+            return MF_ERR_BAD_ARG;
+        if (!c->geom.is_periodic[d] && (region.lo[d] < 0 || region.hi[d] >= c->geom.n_cell[d])) outside = true;
+    }
+    if (!outside) return MF_OK;
+    FillArgs f;
+    std::memset(&f, 0, sizeof(f));
+    f.tab = c->d_tab;
+    f.s = from_array4(view);
+    for (int d = 0; d < 3; ++d) {
+        f.n[d] = c->geom.n_cell[d];
+        f.periodic[d] = c->geom.is_periodic[d];
+        f.lo_bc[d] = c->prm.lo_bc[d];
+        f.hi_bc[d] = c->prm.hi_bc[d];
+        f.prob_lo[d] = c->geom.prob_lo[d];
+        f.dx[d] = c->dx[d];
+    }
+    f.ts = nullptr;
+    f.time_host = time;
+    f.stage = 1;
+    for (int ph = 1; ph <= 3; ++ph) c->ks->fill_region(f, ph, region.lo, region.hi, c->stream);
+    HIPCHK(hipGetLastError());
     return MF_OK;
 }

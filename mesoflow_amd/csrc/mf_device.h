@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "mesoflow_amd.h"
+#include "mf_plugin_api.h"
 
 #define MFD __host__ __device__ __forceinline__
 
@@ -125,6 +126,7 @@ struct Tab : mf_plugin_params {
     double gm1[MF_MAX_SPECIES];
     double cvm[MF_MAX_SPECIES];
     double igm1[MF_MAX_SPECIES];
+    mfp::GeometryData geo;  // the level's bounds and cell sizes (for reference-style plugins)
 };
 
 // tests/Cartesian/ShockTube_N2/{thermo.H:19-133, transport.H, userfuncs.H, chemistry.H, externbc.H}
@@ -164,7 +166,8 @@ struct PShockTubeN2 {
     }
     static constexpr bool kHasFluidSource = false;
     static constexpr bool kHasChemistry = false;
-    static constexpr bool kGammaIgnoresRhoP = true;  // gamma depends on the species only
+    static constexpr bool kGammaIgnoresP = true;  // gamma does not depend on the pressure
+    static constexpr bool kHasBgGas = false;
     template <class C> MFD static void spec_source(const Tab &, const C &, double *src) { src[0] = kZero; }
     MFD static void rate_constants(const Tab &, double, double *) {}
     template <class C> MFD static void spec_source_k(const Tab &T, const double *, const C &cell, double *src) {
@@ -245,7 +248,8 @@ struct PMixture {
         for (int n = 0; n < FLO; ++n) src[n] = kZero;
     }
     static constexpr bool kHasFluidSource = false;
-    static constexpr bool kGammaIgnoresRhoP = true;  // gamma depends on the species only
+    static constexpr bool kGammaIgnoresP = true;  // gamma does not depend on the pressure
+    static constexpr bool kHasBgGas = false;
 };
 
 // models/catalystCylinder: chemistry.H:13-29, externbc.H:13-73
@@ -393,6 +397,88 @@ struct PPorous : PMixture<NSv, NSv - 1> {
         s_ext[RHOE] += kHalf * s_ext[DENS] * (s_ext[VELX] * s_ext[VELX] + s_ext[VELY] * s_ext[VELY] + s_ext[VELZ] * s_ext[VELZ]);
 #pragma unroll
         for (int s = 0; s < NS; ++s) s_ext[FLO + s] = spec[s];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// A case written against mf_plugin_api.h (the reference's signature style)
+// seen through the interface the kernels use: each point-wise function gets a
+// view of its own cell, the level's prob_lo and dx, and the context's table.
+// ---------------------------------------------------------------------------
+template <class U>
+struct PRef {
+    static constexpr int NS = U::NUM_SPECIES;
+    static constexpr int NG = U::NUM_GAS_SPECIES;
+    static constexpr int NV = FLO + NS;
+    static constexpr bool kHasChemistry = U::HAS_CHEMISTRY;
+    static constexpr bool kHasFluidSource = U::HAS_FLUID_SOURCE;
+    static constexpr bool kGammaIgnoresP = U::GAMMA_IGNORES_PRESSURE;
+    static constexpr bool kHasBgGas = U::HAS_BG_GAS;
+    MFD static mfp::GpuArray<double, 3> lo3(const Tab &T) { return {{T.geo.prob_lo[0], T.geo.prob_lo[1], T.geo.prob_lo[2]}}; }
+    MFD static mfp::GpuArray<double, 3> dx3(const Tab &T) { return {{T.geo.dx[0], T.geo.dx[1], T.geo.dx[2]}}; }
+    MFD static double get_r_from_c(const Tab &T, const double *spec) { return U::get_r_from_c(spec, kZero, T); }
+    MFD static double get_r_from_c_bg(const Tab &T, const double *spec, double bg) { return U::get_r_from_c(spec, bg, T); }
+    MFD static double get_bgasconc_from_rc(const Tab &T, double rho, const double *spec) {
+        return U::get_bgasconc_from_rc(rho, spec, T);
+    }
+    MFD static double get_t_from_rpc(const Tab &T, double rho, double pres, const double *spec) {
+        return U::get_t_from_rpc(rho, pres, spec, T);
+    }
+    MFD static double get_p_from_rec(const Tab &T, double rhoe, double rho, const double *spec) {
+        return U::get_p_from_rec(rhoe, rho, spec, T);
+    }
+    MFD static double get_gama_from_rpc(const Tab &T, double rho, double pres, const double *spec) {
+        return U::get_gama_from_rpc(rho, pres, spec, T);
+    }
+    MFD static double get_e_from_rpc(const Tab &T, double rho, double pres, const double *spec) {
+        return U::get_e_from_rpc(rho, pres, spec, T);
+    }
+    MFD static double get_solid_t_from_rhoe(const Tab &T, double rhoe) { return U::get_solid_t_from_rhoe(rhoe, T); }
+    template <class C> MFD static void fluid_transport(const Tab &T, const C &cell, double &visc, double &thcond) {
+        double ph[NV], out[2] = {kZero, kZero};
+#pragma unroll
+        for (int n = 0; n < NV; ++n) ph[n] = cell(n);
+        U::compute_fluid_transport(0, 0, 0, mfp::cell_view(ph, 0, 0, 0), mfp::cell_view(out, 0, 0, 0), lo3(T), dx3(T),
+                                   kZero, T);
+        visc = out[0];
+        thcond = out[1];
+    }
+    template <class C> MFD static void spec_dcoeff(const Tab &T, const C &cell, double *d) {
+        double ph[NV];
+#pragma unroll
+        for (int n = 0; n < NV; ++n) ph[n] = cell(n);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) d[s] = kZero;
+        U::compute_spec_dcoeff(0, 0, 0, mfp::cell_view(ph, 0, 0, 0), mfp::cell_view(d, 0, 0, 0), lo3(T), dx3(T), kZero,
+                               T);
+    }
+    template <class C> MFD static void fluid_source(const Tab &T, const C &cell, double *src) {
+        double ph[NV];
+#pragma unroll
+        for (int n = 0; n < NV; ++n) ph[n] = cell(n);
+        for (int n = 0; n < FLO; ++n) src[n] = kZero;
+        U::compute_fluid_source(0, 0, 0, mfp::cell_view(ph, 0, 0, 0), mfp::cell_view(src, 0, 0, 0), lo3(T), dx3(T),
+                                kZero, T);
+    }
+    template <class C> MFD static void spec_source(const Tab &T, const C &cell, double *src) {
+        double ph[NV], out[NV];
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+            ph[n] = cell(n);
+            out[n] = kZero;
+        }
+        U::compute_spec_source(0, 0, 0, mfp::cell_view(ph, 0, 0, 0), mfp::cell_view(out, 0, 0, 0), lo3(T), dx3(T), kZero,
+                               T);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) src[s] = out[FLO + s];
+    }
+    MFD static void rate_constants(const Tab &, double, double *) {}
+    template <class C> MFD static void spec_source_k(const Tab &T, const double *, const C &cell, double *src) {
+        spec_source(T, cell, src);
+    }
+    MFD static void externalbc(const Tab &T, const double *x, const double *s_int, double *s_ext, int idir, int sgn,
+                               double time) {
+        U::externalbc(x, s_int, s_ext, idir, sgn, time, T.geo, T);
     }
 };
 

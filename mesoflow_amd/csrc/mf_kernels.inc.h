@@ -375,7 +375,7 @@ __global__ __launch_bounds__(256) void k_faces(StageArgs a, Work w) {
 // work, conduction and species diffusion terms are formed first and kept in
 // registers, then each flux component is reconstructed and written once with
 // those terms subtracted in the reference's order. Same arithmetic as
-// face_flux<P,5,D>; the plugin's gamma must depend on the species only (then
+// face_flux<P,5,D>; the plugin's gamma must not depend on the pressure (then
 // gamma(rho_R, presR, spec_R) of the eigenvalue equals the staged gamma).
 // ---------------------------------------------------------------------------
 // component n of cell (i,j,k) as a box sees it: the array value, loaded
@@ -433,7 +433,7 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
     // LDS tile: one record of FS doubles per stencil cell (an odd stride keeps
     // 64-bit reads of consecutive cells on distinct banks)
     constexpr int FVX = NC, FP = NC + 3, FG = NC + 4, FVF = NC + 5, NF = NC + 6, FS = NF | 1;
-    static_assert(P::kGammaIgnoresRhoP, "staged gamma needs a species-only gamma");
+    static_assert(P::kGammaIgnoresP, "the staged gamma needs a gamma that does not depend on the pressure");
     extern __shared__ double lds[];
     if (step_skipped(a.ts)) return;
     const Tab &T = *a.tab;
@@ -1061,9 +1061,65 @@ __global__ __launch_bounds__(256) void k_est(ResArgs r, Work w) {
 // (foextrap, reflect_even/odd for walls: Source/mflo.cpp:70-143) in x,y,z order,
 // then the ext_dir functor of Source/mflo_bcfill.H:16-128 in x,y,z order.
 // ---------------------------------------------------------------------------
+// the rules of one ghost cell iv of a.s for the given phase (cells outside the
+// domain in phase non-periodic directions; phase 0: only in periodic ones)
+template <class P>
+MFD void fill_cell_rules(const FillArgs &a, const int *iv, int phase, const Tab &T, double time) {
+    constexpr int NV = Num<P>::NV;
+    int out_np = 0, out_p = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        bool o = iv[d] < 0 || iv[d] >= a.n[d];
+        if (o) { if (a.periodic[d]) ++out_p; else ++out_np; }
+    }
+    if (phase == 0) {
+        if (out_np != 0 || out_p == 0) return;
+        int s[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            s[d] = iv[d];
+            if (a.periodic[d]) { if (s[d] < 0) s[d] += a.n[d]; else if (s[d] >= a.n[d]) s[d] -= a.n[d]; }
+        }
+        for (int c = 0; c < NV; ++c) a.s(iv[0], iv[1], iv[2], c) = a.s(s[0], s[1], s[2], c);
+        return;
+    }
+    if (out_np != phase) return;
+    for (int d = 0; d < 3; ++d) {
+        if (a.periodic[d]) continue;
+        int side = iv[d] < 0 ? 0 : (iv[d] >= a.n[d] ? 1 : -1);
+        if (side < 0) continue;
+        int bc = side == 0 ? a.lo_bc[d] : a.hi_bc[d];
+        if (bc == MF_BC_EXT_DIR) continue;
+        int s[3] = {iv[0], iv[1], iv[2]};
+        bool refl = (bc == MF_BC_WALL);
+        if (!refl) s[d] = side == 0 ? 0 : a.n[d] - 1;
+        else s[d] = side == 0 ? (-iv[d] - 1) : (2 * (a.n[d] - 1) - iv[d] + 1);
+        for (int c = 0; c < NV; ++c) {
+            double v = a.s(s[0], s[1], s[2], c);
+            bool odd = refl && (c == RHOU || c == RHOV || c == RHOW || c == VELX || c == VELY || c == VELZ);
+            a.s(iv[0], iv[1], iv[2], c) = odd ? -v : v;
+        }
+    }
+    double x[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[d] = a.prob_lo[d] + (iv[d] + 0.5) * a.dx[d];
+    for (int d = 0; d < 3; ++d) {
+        if (a.periodic[d]) continue;
+        int sgn = 0, bnd = 0;
+        if (a.lo_bc[d] == MF_BC_EXT_DIR && iv[d] < 0) { sgn = 1; bnd = 0; }
+        else if (a.hi_bc[d] == MF_BC_EXT_DIR && iv[d] >= a.n[d]) { sgn = -1; bnd = a.n[d] - 1; }
+        if (!sgn) continue;
+        double s_int[NV], s_ext[NV];
+        int loc[3] = {iv[0], iv[1], iv[2]};
+        loc[d] = bnd;
+        for (int c = 0; c < NV; ++c) { s_int[c] = a.s(loc[0], loc[1], loc[2], c); s_ext[c] = 0.0; }
+        P::externalbc(T, x, s_int, s_ext, d, sgn, time);
+        for (int c = 0; c < NV; ++c) a.s(iv[0], iv[1], iv[2], c) = s_ext[c];
+    }
+}
+
 template <class P>
 __global__ __launch_bounds__(256) void k_fill(FillArgs a, int phase) {
-    constexpr int NV = Num<P>::NV;
     if (step_skipped(a.ts)) return;
     const Tab &T = *a.tab;
     const double time = !a.ts ? a.time_host : (a.stage == 1 ? a.ts->t : a.ts->t + a.ts->dt);
@@ -1095,56 +1151,29 @@ __global__ __launch_bounds__(256) void k_fill(FillArgs a, int phase) {
             iv[1] = (int)(r % ny); r /= ny;
             iv[2] = (int)r;
         }
-        int out_np = 0, out_p = 0;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            bool o = iv[d] < 0 || iv[d] >= a.n[d];
-            if (o) { if (a.periodic[d]) ++out_p; else ++out_np; }
-        }
-        if (phase == 0) {
-            if (out_np != 0 || out_p == 0) continue;
-            int s[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                s[d] = iv[d];
-                if (a.periodic[d]) { if (s[d] < 0) s[d] += a.n[d]; else if (s[d] >= a.n[d]) s[d] -= a.n[d]; }
-            }
-            for (int c = 0; c < NV; ++c) a.s(iv[0], iv[1], iv[2], c) = a.s(s[0], s[1], s[2], c);
-            continue;
-        }
-        if (out_np != phase) continue;
-        for (int d = 0; d < 3; ++d) {
-            if (a.periodic[d]) continue;
-            int side = iv[d] < 0 ? 0 : (iv[d] >= a.n[d] ? 1 : -1);
-            if (side < 0) continue;
-            int bc = side == 0 ? a.lo_bc[d] : a.hi_bc[d];
-            if (bc == MF_BC_EXT_DIR) continue;
-            int s[3] = {iv[0], iv[1], iv[2]};
-            bool refl = (bc == MF_BC_WALL);
-            if (!refl) s[d] = side == 0 ? 0 : a.n[d] - 1;
-            else s[d] = side == 0 ? (-iv[d] - 1) : (2 * (a.n[d] - 1) - iv[d] + 1);
-            for (int c = 0; c < NV; ++c) {
-                double v = a.s(s[0], s[1], s[2], c);
-                bool odd = refl && (c == RHOU || c == RHOV || c == RHOW || c == VELX || c == VELY || c == VELZ);
-                a.s(iv[0], iv[1], iv[2], c) = odd ? -v : v;
-            }
-        }
-        double x[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) x[d] = a.prob_lo[d] + (iv[d] + 0.5) * a.dx[d];
-        for (int d = 0; d < 3; ++d) {
-            if (a.periodic[d]) continue;
-            int sgn = 0, bnd = 0;
-            if (a.lo_bc[d] == MF_BC_EXT_DIR && iv[d] < 0) { sgn = 1; bnd = 0; }
-            else if (a.hi_bc[d] == MF_BC_EXT_DIR && iv[d] >= a.n[d]) { sgn = -1; bnd = a.n[d] - 1; }
-            if (!sgn) continue;
-            double s_int[NV], s_ext[NV];
-            int loc[3] = {iv[0], iv[1], iv[2]};
-            loc[d] = bnd;
-            for (int c = 0; c < NV; ++c) { s_int[c] = a.s(loc[0], loc[1], loc[2], c); s_ext[c] = 0.0; }
-            P::externalbc(T, x, s_int, s_ext, d, sgn, time);
-            for (int c = 0; c < NV; ++c) a.s(iv[0], iv[1], iv[2], c) = s_ext[c];
-        }
+        fill_cell_rules<P>(a, iv, phase, T, time);
+    }
+}
+
+// The physical-boundary rules (phases >= 1) over the cells of the region
+// [rlo, rhi] of a.s that lie outside the domain: the ghost region of a box
+// whose other cells are already filled (FillPatchTwoLevels' physical boundary
+// fill of a fine box). Each rule reads only cells of the region: the source
+// planes 0..2 and n-3..n-1 lie inside any box grown by 3 that reaches beyond
+// the domain.
+template <class P>
+__global__ __launch_bounds__(256) void k_fill_region(FillArgs a, int phase, int3 rlo, int3 rhi) {
+    const Tab &T = *a.tab;
+    const double time = a.time_host;
+    const int ex = rhi.x - rlo.x + 1, ey = rhi.y - rlo.y + 1, ez = rhi.z - rlo.z + 1;
+    const int64_t total = (int64_t)ex * ey * ez;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        int iv[3];
+        int64_t r = q;
+        iv[0] = rlo.x + (int)(r % ex); r /= ex;
+        iv[1] = rlo.y + (int)(r % ey); r /= ey;
+        iv[2] = rlo.z + (int)r;
+        fill_cell_rules<P>(a, iv, phase, T, time);
     }
 }
 
@@ -1496,6 +1525,11 @@ __global__ __launch_bounds__(256) void k_chem_split(ChemSplitArgs c) {
         double S[NV];
 #pragma unroll
         for (int n = 0; n < NV; ++n) S[n] = sp[(int64_t)n * sn];
+        if constexpr (P::kHasBgGas) {
+            // store_inertgas_conc (mflo_splitchem.cpp:142-170): the background
+            // gas concentration in the density slot while the chemistry runs
+            if (c.bg && c.store_bg && S[VFRAC] >= kOne) S[DENS] = P::get_bgasconc_from_rc(T, S[RHO], S + FLO);
+        }
         if (P::kHasChemistry && c.nsub > 0) {
             double L[NV];
 #pragma unroll
@@ -1540,7 +1574,13 @@ __global__ __launch_bounds__(256) void k_chem_split(ChemSplitArgs c) {
             for (int s = 0; s < NS; ++s) S[FLO + s] = u[s];
         }
         if (c.finish && S[VFRAC] >= kOne) {
-            const double rho = P::get_r_from_c(T, S + FLO);
+            double rho;
+            if constexpr (P::kHasBgGas) {
+                // update_vars_after_chemsolve with the background gas (mflo_splitchem.cpp:94-97)
+                rho = c.bg ? P::get_r_from_c_bg(T, S + FLO, S[DENS]) : P::get_r_from_c(T, S + FLO);
+            } else {
+                rho = P::get_r_from_c(T, S + FLO);
+            }
             S[RHO] = rho;
             S[DENS] = rho;
             S[RHOU] = rho * S[VELX];
@@ -1572,6 +1612,14 @@ struct Launch {
         if (grid > 65535) grid = 65535;
         if (grid < 1) grid = 1;
         hipLaunchKernelGGL(k_fill<P>, dim3((unsigned)grid), dim3(256), 0, st, a, phase);
+    }
+    static void fill_region(const FillArgs &a, int phase, const int *rlo, const int *rhi, hipStream_t st) {
+        const int64_t total = (int64_t)(rhi[0] - rlo[0] + 1) * (rhi[1] - rlo[1] + 1) * (rhi[2] - rlo[2] + 1);
+        if (total <= 0) return;
+        int64_t grid = (total + 255) / 256;
+        if (grid > 65535) grid = 65535;
+        hipLaunchKernelGGL(k_fill_region<P>, dim3((unsigned)grid), dim3(256), 0, st, a, phase,
+                           make_int3(rlo[0], rlo[1], rlo[2]), make_int3(rhi[0], rhi[1], rhi[2]));
     }
     static void cut(const CutArgs &a, hipStream_t st) {
         if (a.ncut <= 0) return;
@@ -1643,8 +1691,10 @@ struct Launch {
     static KernelSet set(const AdjSet *adj = nullptr) {
         KernelSet k;
         k.ns = P::NS;
+        k.has_bg_gas = P::kHasBgGas;
         k.adj = adj;
         k.fill = &fill;
+        k.fill_region = &fill_region;
         k.cut = &cut;
         k.cut_geom = &cut_geom;
         k.cut_box = &cut_box;

@@ -6,7 +6,9 @@
 // next finer level with the amr.* parameters n_error_buf, blocking_factor,
 // max_grid_size and grid_eff. AMReX is not in the tree; the steps restate its
 // documented algorithm:
-//   1. every tag grows by n_error_buf cells (clipped to the domain);
+//   1. every tag grows by n_error_buf cells, wrapping around the domain in
+//      periodic directions (AMReX maps buffered tags across periodic
+//      boundaries) and clipped to it in the others;
 //   2. tags are coarsened by bf = blocking_factor / ratio (a block is tagged if
 //      any of its cells is), so every box edge lands on the blocking factor;
 //   3. Berger-Rigoutsos clustering of the coarsened tags: a cluster is shrunk
@@ -120,8 +122,9 @@ void cluster(const Grid &g, const IBox &b0, double eff, std::vector<IBox> &acc) 
 
 extern "C" {
 
-int mf_tags_to_boxes(const unsigned char *tags, const int32_t n_cell[3], int n_error_buf, int blocking_factor,
-                     int max_grid_size, double grid_eff, int ratio, mf_box *boxes, int cap, int *nboxes) {
+int mf_tags_to_boxes(const unsigned char *tags, const int32_t n_cell[3], const int32_t *is_periodic, int n_error_buf,
+                     int blocking_factor, int max_grid_size, double grid_eff, int ratio, mf_box *boxes, int cap,
+                     int *nboxes) {
     if (!tags || !n_cell || !nboxes || n_error_buf < 0 || ratio < 1 || blocking_factor < ratio ||
         blocking_factor % ratio != 0 || max_grid_size < blocking_factor || max_grid_size % blocking_factor != 0 ||
         !(grid_eff > 0.0 && grid_eff <= 1.0) || cap < 0 || (cap > 0 && !boxes))
@@ -137,16 +140,33 @@ int mf_tags_to_boxes(const unsigned char *tags, const int32_t n_cell[3], int n_e
     for (int d = 0; d < 3; ++d) g.n[d] = n[d] / bf;
     g.t.assign((size_t)g.n[0] * g.n[1] * g.n[2], 0);
     const int e = n_error_buf;
+    int per[3];
+    for (int d = 0; d < 3; ++d) per[d] = is_periodic ? (is_periodic[d] != 0) : 0;
     for (int k = 0; k < n[2]; ++k)
         for (int j = 0; j < n[1]; ++j)
             for (int i = 0; i < n[0]; ++i) {
                 if (!tags[(size_t)i + (size_t)n[0] * (j + (size_t)n[1] * k)]) continue;
-                const int lo[3] = {std::max(i - e, 0), std::max(j - e, 0), std::max(k - e, 0)};
-                const int hi[3] = {std::min(i + e, n[0] - 1), std::min(j + e, n[1] - 1), std::min(k + e, n[2] - 1)};
-                for (int kc = lo[2] / bf; kc <= hi[2] / bf; ++kc)
-                    for (int jc = lo[1] / bf; jc <= hi[1] / bf; ++jc)
-                        for (int ic = lo[0] / bf; ic <= hi[0] / bf; ++ic)
-                            g.t[(size_t)ic + (size_t)g.n[0] * (jc + (size_t)g.n[1] * kc)] = 1;
+                const int c[3] = {i, j, k};
+                int lo[3], hi[3];
+                for (int d = 0; d < 3; ++d) {
+                    lo[d] = c[d] - e;
+                    hi[d] = c[d] + e;
+                    if (!per[d] || hi[d] - lo[d] + 1 >= n[d]) {  // clip (a buffer wider than a periodic domain covers it)
+                        lo[d] = std::max(lo[d], 0);
+                        hi[d] = std::min(hi[d], n[d] - 1);
+                    }
+                }
+                // buffered cells, each mapped to its periodic image in the domain
+                for (int kk = lo[2]; kk <= hi[2]; ++kk) {
+                    const int kw = ((kk % n[2]) + n[2]) % n[2];
+                    for (int jj = lo[1]; jj <= hi[1]; ++jj) {
+                        const int jw = ((jj % n[1]) + n[1]) % n[1];
+                        for (int ii = lo[0]; ii <= hi[0]; ++ii) {
+                            const int iw = ((ii % n[0]) + n[0]) % n[0];
+                            g.t[(size_t)(iw / bf) + (size_t)g.n[0] * (jw / bf + (size_t)g.n[1] * (kw / bf))] = 1;
+                        }
+                    }
+                }
             }
     // 3: clusters
     std::vector<IBox> cl;

@@ -25,6 +25,8 @@ SIGNATURES = {
     "mf_device_count": ([_P(C.c_int)], C.c_int),
     "mf_ctx_create": ([_P(abi.PluginParams), _P(abi.Params), _P(abi.Geom), _P(C.c_void_p)], C.c_int),
     "mf_ctx_destroy": ([C.c_void_p], C.c_int),
+    "mf_plugin_defaults": ([C.c_int, _P(abi.PluginParams)], C.c_int),
+    "mf_plugin_name": ([C.c_int, _P(C.c_char_p)], C.c_int),
     "mf_ctx_set_plugin_params": ([C.c_void_p, C.c_void_p], C.c_int),
     "mf_ctx_set_stream": ([C.c_void_p, C.c_void_p], C.c_int),
     "mf_ctx_nvars": ([C.c_void_p, _P(C.c_int)], C.c_int),
@@ -47,7 +49,7 @@ SIGNATURES = {
                     C.c_int], C.c_int),
     "mf_level_tag": ([C.c_void_p, C.c_double, _P(C.c_double), _P(C.c_double), _P(C.c_int), C.c_int, C.c_void_p,
                       _P(C.c_int64)], C.c_int),
-    "mf_tags_to_boxes": ([C.c_void_p, _P(C.c_int32), C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _P(abi.Box),
+    "mf_tags_to_boxes": ([C.c_void_p, _P(C.c_int32), _P(C.c_int32), C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _P(abi.Box),
                           C.c_int, _P(C.c_int)], C.c_int),
     "mf_level_create": ([C.c_void_p, _P(abi.Box), C.c_int, _P(C.c_void_p)], C.c_int),
     "mf_level_destroy": ([C.c_void_p], C.c_int),
@@ -103,6 +105,19 @@ def lib():
 def check(st, where):
     if st != abi.MF_OK:
         raise abi.MFError(where, st)
+
+
+def plugin_defaults(kind):
+    """The starting table of a case plugin family (mf_plugin_defaults)."""
+    t = abi.PluginParams()
+    check(lib().mf_plugin_defaults(int(kind), C.byref(t)), "mf_plugin_defaults")
+    return t
+
+
+def plugin_name(kind):
+    n = C.c_char_p()
+    check(lib().mf_plugin_name(int(kind), C.byref(n)), "mf_plugin_name")
+    return n.value.decode()
 
 
 def device_count():

@@ -279,7 +279,8 @@ int orc_box_tag(mf_box bx, mf_array4 state, unsigned char *tags, const double *r
                     const double gx = std::max(std::fabs(s(i + 1, j, k, n) - u), std::fabs(u - s(i - 1, j, k, n)));
                     const double gy = std::max(std::fabs(s(i, j + 1, k, n) - u), std::fabs(u - s(i, j - 1, k, n)));
                     const double gz = std::max(std::fabs(s(i, j, k + 1, n) - u), std::fabs(u - s(i, j, k - 1, n)));
-                    if (std::max(gx, std::max(gy, gz)) > refine_phigrad[c]) t = tv;
+                    // amrex::max(gx, grady, gradz) = max(max(gx, gy), gz) (mflo_tagging.H:66)
+                    if (std::max(std::max(gx, gy), gz) > refine_phigrad[c]) t = tv;
                 }
             }
     return MF_OK;

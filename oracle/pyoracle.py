@@ -59,6 +59,8 @@ def lib():
         L.orc_box_cutcells.restype = C.c_int
         L.orc_box_post.argtypes = [P(abi.PluginParams), P(abi.Params), abi.Box, abi.Array4, C.c_double, C.c_int]
         L.orc_box_post.restype = C.c_int
+        L.orc_fill_physbc.argtypes = [P(abi.PluginParams), P(abi.Params), P(abi.Geom), abi.Box, abi.Array4, C.c_double]
+        L.orc_fill_physbc.restype = C.c_int
         L.orc_kat.argtypes = [P(abi.PluginParams), C.c_int, C.c_void_p, C.c_void_p]
         L.orc_kat.restype = C.c_int
         A4, B = abi.Array4, abi.Box
@@ -255,6 +257,12 @@ def box_tag(bx, state, tags, refine_phi, refine_phigrad, comps, tagval=1):
     gr = (C.c_double * max(n, 1))(*refine_phigrad)
     cp = (C.c_int * max(n, 1))(*comps)
     _ok(lib().orc_box_tag(abi.make_box(*bx), _a4(state), tags.ctypes.data, ph, gr, cp, n, tagval), "orc_box_tag")
+
+
+def fill_physbc(case, region, fab, time):
+    """physical-boundary rules on the cells of region of a numpy fab outside the domain (in place)"""
+    _ok(lib().orc_fill_physbc(C.byref(case.plugin), C.byref(case.params), C.byref(case.geom), abi.make_box(*region),
+                              _a4(fab), float(time)), "orc_fill_physbc")
 
 
 def box_post(case, bx, fab, dt, flags):

@@ -140,6 +140,10 @@ def test_box_tag(ctx):
     nv = ctx.nv
     bx = ((-2, 1, 0), (13, 9, 6))
     st = _rand(rng, tuple(bx[0][d] - 1 for d in range(3)), tuple(bx[1][d] + 1 for d in range(3)), nv)
+    # NaN neighbours: the maxima keep std::max's NaN semantics (the oracle's)
+    a = st[0]
+    for q in range(12):
+        a[7, rng.integers(a.shape[1]), rng.integers(a.shape[2]), rng.integers(a.shape[3])] = np.nan
     phi, grad, comps = [2.5, 1e30, 3.0], [3.9, 4.2, 1e30], [0, 7, nv - 1]
     n = [bx[1][d] - bx[0][d] + 1 for d in range(3)]
     init = np.zeros((n[2], n[1], n[0]), np.uint8)
@@ -175,7 +179,7 @@ def test_level_tag():
     say("level tag: %d tagged (oracle %d)" % (cnt, int(want.sum())))
     assert np.array_equal(got, want) and cnt == int(want.sum()) > 0
     # the regrid of these tags: fine boxes (ratio 2) over every tagged cell
-    boxes = amr.tags_to_boxes(got, 1, 4, 16, 0.8, 2)
+    boxes = amr.tags_to_boxes(got, 1, 4, 16, 0.8, 2, periodic=tuple(c.geom.is_periodic))
     cov = np.zeros((nz * 2, ny * 2, nx * 2), np.int32)
     for lo, hi in boxes:
         cov[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] += 1
@@ -231,25 +235,51 @@ def test_fill_patch_two_levels():
     fctx.close()
 
 
-def test_fill_patch_two_levels_rejects_a_physical_boundary():
-    """a fine box whose ghost region leaves the domain in a non-periodic
-    direction needs the fine level's boundary fill, which this call does not
-    apply: MF_ERR_UNSUPPORTED before any launch"""
-    from mesoflow_amd import abi
-    c = cases.shocktube_n2(n_cell=(32, 32, 32))   # x: ext_dir / wall, y and z periodic
+def test_fill_patch_two_levels_at_physical_boundaries():
+    """fine boxes whose ghost regions leave the domain: x ext_dir (low) / wall
+    (high), y foextrap (low) / wall (high), z periodic. Each box's Sborder
+    against the oracle composition: time interpolation + cell_cons_interp, the
+    fine data and its periodic images, then the physical-boundary rules of the
+    fine level (orc_fill_physbc) on the cells outside the domain, corners
+    included"""
+    from oracle import pyoracle as po
+    c = cases.shocktube_n2(n_cell=(32, 32, 32))
+    c.geom.is_periodic[1] = 0
+    c.params.lo_bc[1], c.params.hi_bc[1] = 2, 6
     fctx = runtime.Context(c)
     nv = fctx.nv
-    rng = np.random.default_rng(2)
-    co = _dev(_rand(rng, (-3, -3, -3), (18, 18, 18), nv))
-    box = ((0, 8, 8), (7, 15, 15))
-    fab = _dev(_rand(rng, *box, nv))
-    sb = _dev(_rand(rng, (-3, 5, 5), (10, 18, 18), nv))
-    with pytest.raises(abi.MFError) as e:
-        amr.fill_patch_two_levels(fctx, box, sb, [(box, fab)], co, co, 0.0, 1.0, 0.5, 2)
-    assert e.value.status == abi.MF_ERR_UNSUPPORTED
-    inner = ((8, 0, 8), (15, 7, 15))   # touches y = 0: periodic, accepted
-    fab2 = _dev(_rand(rng, *inner, nv))
-    sb2 = _dev(_rand(rng, (5, -3, 5), (18, 10, 18), nv))
-    amr.fill_patch_two_levels(fctx, inner, sb2, [(inner, fab2)], co, co, 0.0, 1.0, 0.5, 2)
-    fctx.sync()
+    rng = np.random.default_rng(43)
+    co = _rand(rng, (-3, -3, -3), (18, 18, 18), nv)
+    cn = _rand(rng, (-3, -3, -3), (18, 18, 18), nv)
+    boxes = [((0, 0, 8), (7, 7, 15)), ((24, 24, 28), (31, 31, 31)), ((8, 0, 0), (15, 7, 3)), ((0, 10, 12), (1, 20, 19))]
+    fabs = [_rand(rng, *b, nv) for b in boxes]
+    dfabs = [(b, _dev(f)) for b, f in zip(boxes, fabs)]
+    dco, dcn = _dev(co), _dev(cn)
+    N = 32
+    for q, (lo, hi) in enumerate(boxes):
+        glo, ghi = tuple(v - 3 for v in lo), tuple(v + 3 for v in hi)
+        clo, chi = tuple(v // 2 - 1 for v in glo), tuple(v // 2 + 1 for v in ghi)
+        ct = _rand(rng, clo, chi, nv)
+        po.time_interp((clo, chi), ct, co, cn, 0.0, 1.0, 0.25)
+        want = _rand(rng, glo, ghi, nv)
+        po.interp_cons((glo, ghi), ct, want, 2)
+        w = want[0]
+        for (blo, bhi), (fa, _) in zip(boxes, fabs):
+            for sh in [(0, 0, c_) for c_ in (-N, 0, N)]:             # periodic in z only
+                xl = [max(glo[d], blo[d] + sh[d]) for d in range(3)]
+                xh = [min(ghi[d], bhi[d] + sh[d]) for d in range(3)]
+                if any(xh[d] < xl[d] for d in range(3)):
+                    continue
+                w[:, xl[2] - glo[2]:xh[2] - glo[2] + 1, xl[1] - glo[1]:xh[1] - glo[1] + 1,
+                  xl[0] - glo[0]:xh[0] - glo[0] + 1] = \
+                    fa[:, xl[2] - sh[2] - blo[2]:xh[2] - sh[2] - blo[2] + 1,
+                       xl[1] - sh[1] - blo[1]:xh[1] - sh[1] - blo[1] + 1,
+                       xl[0] - sh[0] - blo[0]:xh[0] - sh[0] - blo[0] + 1]
+        po.fill_physbc(c, (glo, ghi), want, 0.25)
+        dsb = _dev(_rand(rng, glo, ghi, nv))
+        amr.fill_patch_two_levels(fctx, (lo, hi), dsb, dfabs, dco, dcn, 0.0, 1.0, 0.25, 2)
+        got = _host(dsb, fctx)
+        nout = sum(1 for d in range(2) if glo[d] < 0 or ghi[d] >= N)
+        say("fill patch at the boundary, box %d (%d non-periodic sides crossed): bitwise %s" % (q, nout, _same(got, w)))
+        assert _same(got, w)
     fctx.close()

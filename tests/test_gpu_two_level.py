@@ -314,7 +314,7 @@ def test_regrid_cycle_remakes_the_fine_level():
     Lc = runtime.Level(ctx_c)
     Lc.set_state(s0)
     tags, ntag = amr.level_tag(Lc, 0.0, [1e30], [0.1], [abi.DENS])
-    new_boxes = amr.tags_to_boxes(tags, 1, 4, 16, 0.7, R)
+    new_boxes = amr.tags_to_boxes(tags, 1, 4, 16, 0.7, R, periodic=(1, 1, 1))
     say("regrid: %d tagged coarse cells -> %d fine boxes" % (ntag, len(new_boxes)))
     assert ntag > 0 and new_boxes
     # the old fine level: one box of interpolated coarse data plus a perturbation

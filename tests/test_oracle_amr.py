@@ -251,6 +251,30 @@ def test_tagging_ghost_layer_and_directions():
         assert np.array_equal(t, want), d
 
 
+def test_tagging_nan_neighbour_follows_std_max():
+    """amrex::max is std::max, (a < b) ? b : a: a NaN first argument propagates,
+    a NaN second argument is dropped, and amrex::max(gx, gy, gz) is
+    max(max(gx, gy), gz) (mflo_tagging.H:46-66). With one NaN neighbour of the
+    cell and one large difference elsewhere, the cell is tagged exactly when the
+    large difference survives: a NaN at x+ makes gx NaN and the whole maximum
+    NaN (never tagged); a NaN at y+ (z+) drops the y (z) differences; a NaN at
+    x-, y-, z- drops only its own difference."""
+    lo, hi = (-1, -1, -1), (1, 1, 1)
+    bx = ((0, 0, 0), (0, 0, 0))
+    pos = {"x+": (1, 0, 0), "x-": (-1, 0, 0), "y+": (0, 1, 0), "y-": (0, -1, 0), "z+": (0, 0, 1), "z-": (0, 0, -1)}
+    killed = {"x+": set(pos), "x-": {"x-"}, "y+": {"y+", "y-"}, "y-": {"y-"}, "z+": {"z+", "z-"}, "z-": {"z-"}}
+    for pn, qn in itertools.product(pos, pos):
+        if pn == qn:
+            continue
+        st = _fab(lo, hi, 1)
+        a = st[0]
+        p, q = pos[pn], pos[qn]
+        a[0, 1 + q[2], 1 + q[1], 1 + q[0]] = 2.0
+        a[0, 1 + p[2], 1 + p[1], 1 + p[0]] = np.nan
+        t = _tag(st, bx, [1e30], [1.0], [0])
+        assert bool(t[0, 0, 0]) == (qn not in killed[pn]), (pn, qn)
+
+
 def test_box_post_primitives_idempotent_and_clip_exact():
     """orc_box_post: on a state whose primitives are consistent, the primitive
     update changes nothing beyond rounding (a second pass nothing at all);
@@ -277,3 +301,35 @@ def test_box_post_primitives_idempotent_and_clip_exact():
     assert np.all(e[abi.FLO_NVARS:][neg] == 0.0)
     assert np.array_equal(e[abi.FLO_NVARS:][~neg], d[abi.FLO_NVARS:][~neg])
     assert np.array_equal(e[:abi.FLO_NVARS], d[:abi.FLO_NVARS])
+
+
+def test_physical_boundary_fill_of_a_box_region_matches_the_level_fill():
+    """orc_fill_physbc (the physical-boundary part of FillPatchTwoLevels for a
+    fine box) on a box's ghost region, with every other cell of the region
+    taken from the level fill, reproduces the level's own ghost fill (the
+    FillPatchSingleLevel restatement pinned by the ghost-cell rules) on the
+    cells outside the domain: faces, edges and corners, ext_dir, foextrap and
+    walls."""
+    from mesoflow_amd import cases
+    from oracle.pyoracle import OracleLevel
+    c = cases.shocktube_n2(n_cell=(12, 10, 8), prob_hi=(1.0, 10 / 12, 8 / 12))
+    c.geom.is_periodic[1] = 0
+    c.params.lo_bc[1], c.params.hi_bc[1] = 2, 6
+    rng = np.random.default_rng(9)
+    s = c.make_state()
+    s *= 1.0 + 0.1 * rng.uniform(-1, 1, s.shape)
+    L = OracleLevel(c)
+    L.set_state(s)
+    pad = L.fill_ghosts(0.37)             # [nv][nz+6][ny+6][nx+6], index -3 at 0
+    n = c.n_cell
+    for lo, hi in (((0, 0, 2), (4, 3, 5)), ((8, 6, 0), (11, 9, 7)), ((0, 7, 0), (11, 9, 2)), ((3, 2, 2), (6, 5, 5))):
+        glo, ghi = tuple(v - 3 for v in lo), tuple(v + 3 for v in hi)
+        sl = tuple([slice(None)] + [slice(glo[d] + 3, ghi[d] + 4) for d in (2, 1, 0)])
+        want = pad[sl].copy()
+        fab = want.copy()
+        k, j, i = np.meshgrid(np.arange(glo[2], ghi[2] + 1), np.arange(glo[1], ghi[1] + 1),
+                              np.arange(glo[0], ghi[0] + 1), indexing="ij")
+        outside = (i < 0) | (i >= n[0]) | (j < 0) | (j >= n[1])      # x and y are not periodic
+        fab[:, outside] = np.nan
+        po.fill_physbc(c, (glo, ghi), (fab, glo), 0.37)
+        assert np.array_equal(fab, want), (lo, hi)

@@ -65,3 +65,29 @@ def test_empty_tags_and_bad_arguments():
         assert e.value.status == abi.MF_ERR_BAD_ARG
     with pytest.raises(abi.MFError):
         amr.tags_to_boxes(np.zeros((8, 8, 6), np.uint8), 1, 8, 16, 0.7, 2)   # 6 is not a multiple of 8/2
+
+
+def test_buffer_wraps_across_periodic_seams():
+    """AMReX maps buffered tags across periodic boundaries: a tag at i = 0 in a
+    periodic direction also buffers the cells at i = n-1 (and likewise at the
+    high side); in a non-periodic direction the buffer is clipped."""
+    n = 16
+    tags = np.zeros((n, n, n), np.uint8)
+    tags[8, 8, 0] = 1                      # at the low x face
+    tags[8, n - 1, 8] = 1                  # at the high y face
+    e, bf, mg, r = 1, 2, 32, 2
+    boxes = amr.tags_to_boxes(tags, e, bf, mg, 0.7, r, periodic=(1, 1, 0))
+    cov = _check(tags, boxes, e, bf, mg, r)
+    fine_cov = lambda i, j, k: cov[k * r, j * r, i * r] == 1
+    assert fine_cov(n - 1, 8, 8) and fine_cov(n - 1, 7, 9)        # x image of the first tag's buffer
+    assert fine_cov(8, 0, 8) and fine_cov(9, 0, 7)                # y image of the second tag's buffer
+    # without periodicity the images are not buffered
+    boxes = amr.tags_to_boxes(tags, e, bf, mg, 0.7, r)
+    cov = _check(tags, boxes, e, bf, mg, r)
+    assert not fine_cov(n - 1, 8, 8) and not fine_cov(8, 0, 8)
+    # z is not periodic: a tag at k = 0 is clipped there, not wrapped
+    tags2 = np.zeros((n, n, n), np.uint8)
+    tags2[0, 4, 4] = 1
+    boxes = amr.tags_to_boxes(tags2, e, bf, mg, 0.7, r, periodic=(1, 1, 0))
+    cov = _check(tags2, boxes, e, bf, mg, r)
+    assert not fine_cov(4, 4, n - 1)
