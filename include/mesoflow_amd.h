@@ -193,6 +193,20 @@ int mf_box_cutcells(void *ctx, mf_box bx, mf_array4 sborder, double time);
 int mf_box_rhs(void *ctx, mf_box bx, mf_array4 sborder, mf_array4 dsdt,
                const mf_array4 *flux3, double time);
 
+/* mf_box_rhs without flux views that allocates nothing. No counterpart in the
+ * reference: for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation). The face
+ * fluxes go to a scratch array of 8M doubles (64 MiB) in the library's device
+ * memory, one per GPU, held until the process ends; a box needs
+ * 3 (n_x+1) n_y n_z (5 + NUM_SPECIES) doubles (a 64^3 box with up to 5
+ * species, a 32^3 box with any), a larger one returns MF_ERR_TOO_LARGE. Each
+ * call writes every scratch value it reads, so nothing carries over between
+ * calls; but every call on a GPU shares that GPU's one array, so calls that
+ * overlap on one GPU (other streams or host threads) race. The context's
+ * device must be the current one (MF_ERR_BAD_ARG otherwise). Same results as
+ * mf_box_rhs(ctx, bx, sborder, dsdt, NULL, time). */
+int mf_box_rhs_noalloc(void *ctx, mf_box bx, mf_array4 sborder, mf_array4 dsdt, double time);
+
 /* The MultiFab arithmetic of one SSP-RK3 stage on the valid box
  * (Source/mflo_evolve.cpp:283, 296-298, 309-311), same operation order:
  *   stage 1: s_new = 1*src + dt*dsdt                    (src = Sborder)

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <new>
+#include <mutex>
 #include <vector>
 #include "mf_args.h"
 
@@ -37,6 +38,13 @@ const CaseFamily *find_case_family(int id) {
 }
 }  // namespace mfd
 
+// Scratch of mf_box_rhs_noalloc: the face fluxes of one box, in a file-scope
+// device array that each GPU holds from the library's load to the end of the
+// process. Host code uses only the address hipGetSymbolAddress returns for
+// the current device (noalloc_ptr below), never the array's name.
+static constexpr size_t kNoAllocDoubles = (size_t)8 << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
 namespace {
 
 #define HIPCHK(x)                                      \
@@ -50,6 +58,28 @@ namespace {
 
 thread_local hipError_t last_hip_error = hipSuccess;
 
+constexpr int kMaxDevices = 64;
+std::mutex noalloc_mu;
+double *noalloc_ptr[kMaxDevices] = {};
+int noalloc_state[kMaxDevices] = {};  // 0 not resolved, 1 resolved, -1 failed
+
+// resolves the scratch array's address on the current device once
+int resolve_noalloc(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) return MF_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (noalloc_state[dev] == 0) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            noalloc_state[dev] = -1;
+        } else {
+            noalloc_ptr[dev] = (double *)p;
+            noalloc_state[dev] = 1;
+        }
+    }
+    return noalloc_state[dev] == 1 ? MF_OK : MF_ERR_HIP;
+}
 
 struct Ctx {
     mf_plugin_params tab;
@@ -1173,6 +1203,9 @@ int mf_ctx_create(const mf_plugin_params *plugin, const mf_params *params, const
         delete c;
         return MF_ERR_HIP;
     }
+    // the no-allocation entry point's scratch on this device (a failure here
+    // only disables mf_box_rhs_noalloc, which then returns MF_ERR_HIP)
+    (void)resolve_noalloc(c->device);
     *ctx_out = c;
     return MF_OK;
 }
@@ -1375,6 +1408,16 @@ int box_rhs_impl(Ctx *c, mf_box bx, mf_array4 sborder, mf_array4 dsdt, const mf_
 
 int mf_box_rhs(void *ctx, mf_box bx, mf_array4 sborder, mf_array4 dsdt, const mf_array4 *flux3, double time) {
     return box_rhs_impl((Ctx *)ctx, bx, sborder, dsdt, flux3, time, nullptr, 0);
+}
+
+int mf_box_rhs_noalloc(void *ctx, mf_box bx, mf_array4 sborder, mf_array4 dsdt, double time) {
+    Ctx *c = (Ctx *)ctx;
+    if (!c) return MF_ERR_BAD_ARG;
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != c->device) return MF_ERR_BAD_ARG;  // the context's device must be current
+    if (dev < 0 || dev >= kMaxDevices || noalloc_state[dev] != 1) return MF_ERR_HIP;
+    return box_rhs_impl(c, bx, sborder, dsdt, nullptr, time, noalloc_ptr[dev], kNoAllocDoubles);
 }
 
 int mf_box_rk3_combine(void *ctx, int stage, mf_box bx, mf_array4 s_new, mf_array4 src, mf_array4 dsdt, double dt) {

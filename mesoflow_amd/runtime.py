@@ -33,6 +33,7 @@ SIGNATURES = {
     "mf_ctx_sync": ([C.c_void_p], C.c_int),
     "mf_box_cutcells": ([C.c_void_p, abi.Box, abi.Array4, C.c_double], C.c_int),
     "mf_box_rhs": ([C.c_void_p, abi.Box, abi.Array4, abi.Array4, _P(abi.Array4), C.c_double], C.c_int),
+    "mf_box_rhs_noalloc": ([C.c_void_p, abi.Box, abi.Array4, abi.Array4, C.c_double], C.c_int),
     "mf_box_rk3_combine": ([C.c_void_p, C.c_int, abi.Box, abi.Array4, abi.Array4, abi.Array4, C.c_double], C.c_int),
     "mf_box_post": ([C.c_void_p, abi.Box, abi.Array4, C.c_double, C.c_int], C.c_int),
     "mf_box_average_down": ([C.c_void_p, abi.Box, abi.Array4, abi.Array4, C.c_int], C.c_int),
