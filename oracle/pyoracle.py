@@ -63,6 +63,22 @@ def lib():
         L.orc_fill_physbc.restype = C.c_int
         L.orc_kat.argtypes = [P(abi.PluginParams), C.c_int, C.c_void_p, C.c_void_p]
         L.orc_kat.restype = C.c_int
+        L.orc_hier_create.argtypes = [P(abi.PluginParams), C.c_void_p, C.c_int, P(C.c_void_p)]
+        L.orc_hier_create.restype = C.c_int
+        L.orc_hier_destroy.argtypes = [C.c_void_p]
+        L.orc_hier_add_level.argtypes = [C.c_void_p, P(abi.Box), C.c_int, C.c_void_p]
+        L.orc_hier_add_level.restype = C.c_int
+        L.orc_hier_regrid.argtypes = [C.c_void_p, P(abi.Box), C.c_int, C.c_double]
+        L.orc_hier_regrid.restype = C.c_int
+        L.orc_hier_step.argtypes = [C.c_void_p, C.c_int, C.c_int, P(abi.Box), C.c_int]
+        L.orc_hier_step.restype = C.c_int
+        L.orc_hier_level.argtypes = [C.c_void_p, C.c_int]
+        L.orc_hier_level.restype = C.c_void_p
+        L.orc_hier_nlevels.argtypes = [C.c_void_p]
+        L.orc_hier_nlevels.restype = C.c_int
+        L.orc_hier_info.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int), C.c_void_p]
+        L.orc_hier_set_only_flow.argtypes = [C.c_void_p, C.c_int]
+        L.orc_hier_set_reflux.argtypes = [C.c_void_p, C.c_int]
         A4, B = abi.Array4, abi.Box
         for name, args in (("orc_average_down", [B, A4, A4, C.c_int, C.c_int]),
                            ("orc_time_interp", [B, A4, A4, A4, C.c_double, C.c_double, C.c_double, C.c_int]),
@@ -155,6 +171,77 @@ class OracleLevel:
 
     def set_time(self, t, dt_prev, istep):
         lib().orc_level_set_time(self.h, float(t), float(dt_prev), int(istep))
+
+
+def _box_array(boxes):
+    boxes = list(boxes)
+    arr = (abi.Box * max(len(boxes), 1))(*[abi.make_box(lo, hi) for lo, hi in boxes])
+    return arr, len(boxes)
+
+
+class OracleHier:
+    """The oracle's two-level hierarchy (mf_hier_oracle.h) over an OracleLevel
+    as level 0: EvolveAMR's step with subcycling, flux registers, reflux,
+    average_down and regrid."""
+
+    def __init__(self, level0, ratio=2):
+        self.L0 = level0
+        self.ratio = ratio
+        self.nv = level0.nv
+        h = C.c_void_p()
+        _ok(lib().orc_hier_create(C.byref(level0.case.plugin), level0.h, int(ratio), C.byref(h)), "orc_hier_create")
+        self.h = h
+        n = level0.case.n_cell
+        self.fine_shape = (self.nv,) + tuple(q * ratio for q in n[::-1])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_hier_destroy(self.h)
+            self.h = None
+
+    def add_level(self, boxes, state):
+        """MakeNewLevelFromScratch: level 1 with the boxes ((lo, hi) in fine
+        cells) and its initial data, dense over the fine domain"""
+        s = np.ascontiguousarray(state, dtype=np.float64)
+        assert s.shape == self.fine_shape
+        arr, nb = _box_array(boxes)
+        _ok(lib().orc_hier_add_level(self.h, arr, nb, _ptr(s)), "orc_hier_add_level")
+
+    def regrid(self, boxes, time):
+        arr, nb = _box_array(boxes)
+        _ok(lib().orc_hier_regrid(self.h, arr, nb, float(time)), "orc_hier_regrid")
+
+    def step(self, n=1, regrid_boxes=None):
+        """n coarse steps; regrid_boxes: a regrid of level 1 at the start of the first"""
+        if regrid_boxes is None:
+            arr, nb = _box_array([])
+            _ok(lib().orc_hier_step(self.h, int(n), 0, arr, 0), "orc_hier_step")
+        else:
+            arr, nb = _box_array(regrid_boxes)
+            _ok(lib().orc_hier_step(self.h, int(n), 1, arr, nb), "orc_hier_step")
+
+    def nlevels(self):
+        return lib().orc_hier_nlevels(self.h)
+
+    def get_state(self, lev):
+        p = lib().orc_hier_level(self.h, int(lev))
+        if not p:
+            raise abi.MFError("orc_hier_level", abi.MF_ERR_BAD_ARG)
+        out = np.empty(self.L0.shape if lev == 0 else self.fine_shape)
+        lib().orc_level_get_state(p, _ptr(out))
+        return out
+
+    def info(self):
+        t, dt, ist = C.c_double(), C.c_double(), C.c_int()
+        res = np.zeros(self.nv)
+        lib().orc_hier_info(self.h, C.byref(t), C.byref(dt), C.byref(ist), _ptr(res))
+        return t.value, dt.value, ist.value, res
+
+    def set_only_flow(self, flag):
+        lib().orc_hier_set_only_flow(self.h, int(bool(flag)))
+
+    def set_reflux(self, flag):
+        lib().orc_hier_set_reflux(self.h, int(bool(flag)))
 
 
 def kat(plugin, func, inp, nout):
