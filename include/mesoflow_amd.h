@@ -390,6 +390,61 @@ int mf_tags_to_boxes(const unsigned char *tags, const int32_t n_cell[3], const i
  * query the count; cap is the capacity of boxes). */
 int mf_level_boxes(void *level, mf_box *boxes, int cap, int *nboxes, mf_geom *geom, int *nv);
 
+/* ------------------------------------------------------------------------ */
+/* Two-level hierarchy: mflo's AmrCore levels 0 and 1                        */
+/* ------------------------------------------------------------------------ */
+/* A hierarchy over a level 0 (created by mf_level_create, which keeps its
+ * state and time): level 1 refines it by ratio (2..4), with subcycling
+ * (nsubsteps = ratio, Source/mflo.cpp:56-61). While the hierarchy exists the
+ * level-0 handle is stepped only through it (its single-level step and
+ * evolve calls return MF_ERR_UNSUPPORTED) and mf_level_destroy refuses it;
+ * mf_hier_destroy releases level 1 and returns level 0 to its caller. */
+int mf_hier_create(void *level0, int ratio, void **hier_out);
+int mf_hier_destroy(void *hier);
+/* MakeNewLevelFromScratch (Source/mflo_gridwork.cpp:91-123) with the
+ * caller's initial data: level 1 over the boxes (fine indices, inside the
+ * refined domain, aligned to the ratio, disjoint), state dense over the whole
+ * fine domain [nv][nz*r][ny*r][nx*r] (only the boxes' cells are used), host or
+ * device memory. *level_out (may be NULL) receives the level's handle: its
+ * state, boxes, plotfile and checkpoint calls work as for level 0, its state
+ * view holds the valid cells of its boxes. */
+int mf_hier_add_level(void *hier, const mf_box *boxes, int nboxes, const double *state, int is_device,
+                      void **level_out);
+/* Handle and count of the hierarchy's levels. */
+int mf_hier_level(void *hier, int lev, void **level_out);
+int mf_hier_nlevels(void *hier, int *nlev);
+/* mflo.do_reflux (default 1): Reflux of level 0 after the fine substeps. */
+int mf_hier_set_reflux(void *hier, int do_reflux);
+/* Regrid every regrid_int coarse steps (0: never; Source/mflo_evolve.cpp:
+ * 177-209): at the start of a step with istep > the last regrid and istep %
+ * regrid_int == 0, after ComputeDt, the tags of level 0 (mf_level_tag with
+ * the ErrorEst criteria refine_phi / refine_phigrad of the components comps)
+ * give the new level-1 boxes (mf_tags_to_boxes with amr.n_error_buf,
+ * blocking_factor (a multiple of the ratio), max_grid_size, grid_eff); level
+ * 1 is then remade (RemakeLevel: its data where the old boxes covered, the
+ * coarse level interpolated elsewhere), made (MakeNewLevelFromCoarse, dt[1] =
+ * dt[0] / ratio) or cleared (no tags). */
+int mf_hier_set_regrid(void *hier, int regrid_int, const double *refine_phi, const double *refine_phigrad,
+                       const int *comps, int ntag, int n_error_buf, int blocking_factor, int max_grid_size,
+                       double grid_eff);
+/* The same regrid now, with the caller's boxes for level 1 (none: clear it). */
+int mf_hier_regrid(void *hier, const mf_box *boxes, int nboxes);
+/* EvolveAMR's step over the hierarchy (Source/mflo_evolve.cpp:20-82, no I/O),
+ * nsteps coarse steps: ComputeDt over the levels (Source/mflo_timestep.cpp:
+ * 18-50), timeStep's recursion (:175-249): Advance_coupled_strang of level 0
+ * (with CrseInit of its fluxes at each stage), ratio substeps of level 1
+ * (FillPatchTwoLevels at each stage, FineAdd, its chemistry at every substep:
+ * the recursion passes no only_flow), Reflux and AverageDownTo; then
+ * update_primitive_vars and clip on both levels and the residual norms of
+ * level 0. mode as mf_level_step (per box, one persistent launch per level,
+ * substep and stage, or the whole coarse step replayed from a hipGraph). */
+int mf_hier_step(void *hier, int nsteps, int mode);
+/* EvolveAMR(final_time, only_flow) over the hierarchy (only_flow applies to
+ * level 0, as in the reference). */
+int mf_hier_evolve(void *hier, double final_time, int max_step, int only_flow, int mode, int *nsteps_out);
+/* Time, dt[0], coarse step count and residual norms (level 0). */
+int mf_hier_info(void *hier, double *time, double *dt, int *istep, double *resnorms);
+
 /* Output of a level (Source/mflo_plot.cpp). varnames: nv names
  * (Source/mflo.cpp:34-53 order), or NULL for rho, rhou, rhov, rhow, rhoe,
  * density, velx, vely, velz, pressure, temperature, volfrac, S0, S1, ..

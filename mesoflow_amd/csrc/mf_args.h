@@ -199,6 +199,10 @@ struct KernelSet {
     void (*fill)(const FillArgs &, int phase, hipStream_t);
     // physical-boundary phase of the fill over the region [rlo, rhi] of a.s
     void (*fill_region)(const FillArgs &, int phase, const int *rlo, const int *rhi, hipStream_t);
+    // physical-boundary phase of the fill over a list of cells ([3][n])
+    void (*fill_list)(const FillArgs &, int phase, const int *list, int n, hipStream_t);
+    // chemistry alone over the tile list (st.src -> dst)
+    void (*chem)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
     void (*cut)(const CutArgs &, hipStream_t);
     void (*cut_geom)(const CutArgs &, double *geom, unsigned *mask, hipStream_t);
     void (*cut_box)(const CutArgs &, hipStream_t);

@@ -160,6 +160,12 @@ struct Level {
     hipGraphExec_t gexec = nullptr;
     struct AdjState *adj = nullptr;
     int post_slots = 1;  // partial-sum slots written by the post-step pass (one per block)
+    // a level of a hierarchy (mf_hier_*): `partial` for a level above 0 whose
+    // boxes cover part of its domain (its ghost cells come from the two-level
+    // fill; its cut list holds the cut cells of its boxes only); `hier` the
+    // owning hierarchy, whose driver alone steps the level
+    bool partial = false;
+    void *hier = nullptr;
 };
 
 // Discrete adjoint of a fixed-dt run of nsteps steps (mf_adjoint_forward /
@@ -185,6 +191,9 @@ struct AdjState {
     unsigned *nbrmask = nullptr;
     int nnbr = 0, geom_gen = -1;
 };
+
+int level_alloc(Ctx *c, const mf_box *boxes, int nboxes, bool partial, Level **out);
+int level_free(Level *L);
 
 const KernelSet *pick_kernels(const mf_plugin_params &t) {
     if (const CaseFamily *f = find_case_family(t.kind)) return t.nspec == f->nspec ? f->ks : nullptr;
@@ -507,10 +516,18 @@ int build_cutlist(Level *L, const double *vfrac_host) {
     const int nx = L->n[0], ny = L->n[1], nz = L->n[2];
     const int64_t ncell = (int64_t)nx * ny * nz;
     std::vector<int> id(ncell, -1), li, lj, lk;
+    std::vector<unsigned char> cov;
+    if (L->partial) {
+        cov.assign((size_t)ncell, 0);
+        for (const mf_box &B : L->boxes)
+            for (int k = B.lo[2]; k <= B.hi[2]; ++k)
+                for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+                    std::memset(cov.data() + (size_t)nx * (j + (size_t)ny * k) + B.lo[0], 1, (size_t)(B.hi[0] - B.lo[0] + 1));
+    }
     int nc = 0;
     for (int64_t q = 0; q < ncell; ++q) {
         double v = vfrac_host[q];
-        if (v > 0.0 && v < 1.0) {
+        if (v > 0.0 && v < 1.0 && (cov.empty() || cov[(size_t)q])) {
             id[q] = nc++;
             li.push_back((int)(q % nx));
             lj.push_back((int)((q / nx) % ny));
@@ -536,12 +553,16 @@ int build_cutlist(Level *L, const double *vfrac_host) {
         HIPCHK(hipMalloc(&L->cutbuf, sizeof(double) * (size_t)nc * L->c->nv));
         HIPCHK(hipMalloc(&L->cutgeom, sizeof(double) * 6 * (size_t)nc));
         HIPCHK(hipMalloc(&L->cutmask, sizeof(unsigned) * 4 * (size_t)nc));
-        // the geometry reads vfrac of the 5^3 neighbourhoods: fill the ghost layers first
-        FillArgs f = fill_args(L, L->arr[0], 1);
-        f.ts = nullptr;
-        f.time_host = 0.0;
-        for (int ph = 0; ph <= 3; ++ph) L->c->ks->fill(f, ph, L->c->stream);
-        L->c->ks->cut_geom(cut_args(L, L->arr[0]), L->cutgeom, L->cutmask, L->c->stream);
+        // the geometry reads vfrac of the 5^3 neighbourhoods: fill the ghost
+        // layers first (a partly covered level recomputes it after each of its
+        // two-level fills, whose ghost vfrac comes from the coarse level)
+        if (!L->partial) {
+            FillArgs f = fill_args(L, L->arr[0], 1);
+            f.ts = nullptr;
+            f.time_host = 0.0;
+            for (int ph = 0; ph <= 3; ++ph) L->c->ks->fill(f, ph, L->c->stream);
+            L->c->ks->cut_geom(cut_args(L, L->arr[0]), L->cutgeom, L->cutmask, L->c->stream);
+        }
         HIPCHK(hipGetLastError());
     }
     if (L->gexec) { hipGraphExecDestroy(L->gexec); L->gexec = nullptr; }
@@ -1462,9 +1483,24 @@ int mf_level_create(void *ctx, const mf_box *boxes, int nboxes, void **level_out
     Ctx *c = (Ctx *)ctx;
     if (!c || !level_out || nboxes < 0 || (nboxes > 0 && !boxes)) return MF_ERR_BAD_ARG;
     *level_out = nullptr;
+    Level *L = nullptr;
+    int st = level_alloc(c, boxes, nboxes, false, &L);
+    if (st) return st;
+    *level_out = L;
+    return MF_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// a level over the context's domain: boxes that tile it (partial: disjoint
+// boxes inside it, a level above 0 of a hierarchy)
+int level_alloc(Ctx *c, const mf_box *boxes, int nboxes, bool partial, Level **out) {
+    *out = nullptr;
     Level *L = new (std::nothrow) Level();
     if (!L) return MF_ERR_BAD_ARG;
     L->c = c;
+    L->partial = partial;
     for (int d = 0; d < 3; ++d) L->n[d] = c->geom.n_cell[d];
     const int64_t ncell = (int64_t)L->n[0] * L->n[1] * L->n[2];
     if (nboxes == 0) {
@@ -1480,8 +1516,8 @@ int mf_level_create(void *ctx, const mf_box *boxes, int nboxes, void **level_out
                    (boxes[b].hi[2] - boxes[b].lo[2] + 1);
             L->boxes.push_back(boxes[b]);
         }
-        if (cov != ncell) { delete L; return MF_ERR_BAD_ARG; }  // boxes must tile the domain
-        if (nboxes > 1) {
+        if (!partial && cov != ncell) { delete L; return MF_ERR_BAD_ARG; }  // boxes must tile the domain
+        if (nboxes > 1 || partial) {
             // disjoint (with the volume check: they cover the domain); overlapping
             // boxes would have tiles writing the same cells and leave a gap unupdated
             std::vector<unsigned char> mark((size_t)ncell, 0);
@@ -1511,7 +1547,7 @@ int mf_level_create(void *ctx, const mf_box *boxes, int nboxes, void **level_out
     ok = ok && hipMalloc(&L->cutid, sizeof(int) * ncell) == hipSuccess;
     ok = ok && hipMalloc(&L->ts, sizeof(TimeState)) == hipSuccess;
     if (!ok) {
-        mf_level_destroy(L);
+        level_free(L);
         return MF_ERR_HIP;
     }
     for (int q = 0; q < 3; ++q) {
@@ -1528,14 +1564,24 @@ int mf_level_create(void *ctx, const mf_box *boxes, int nboxes, void **level_out
     h.active = 1;
     hipMemcpy(L->ts, &h, sizeof(h), hipMemcpyHostToDevice);
     int st = rebuild_tiles(L);
-    if (st) { mf_level_destroy(L); return st; }
-    *level_out = L;
+    if (st) { level_free(L); return st; }
+    *out = L;
     return MF_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int mf_level_destroy(void *level) {
     Level *L = (Level *)level;
-    if (!L) return MF_ERR_BAD_ARG;
+    if (!L || L->hier) return MF_ERR_BAD_ARG;  // a level of a hierarchy goes with the hierarchy
+    return level_free(L);
+}
+
+}  // extern "C"
+
+namespace {
+int level_free(Level *L) {
     if (L->c) hipStreamSynchronize(L->c->stream);
     if (L->gexec) hipGraphExecDestroy(L->gexec);
     if (L->graph) hipGraphDestroy(L->graph);
@@ -1554,6 +1600,9 @@ int mf_level_destroy(void *level) {
     delete L;
     return MF_OK;
 }
+}  // namespace
+
+extern "C" {
 
 int mf_level_set_state(void *level, const double *state, int is_device) {
     Level *L = (Level *)level;
@@ -1622,6 +1671,7 @@ int mf_level_state_view(void *level, mf_array4 *v) {
 int mf_fill_ghosts_single_level(void *level, double time) {
     Level *L = (Level *)level;
     if (!L) return MF_ERR_BAD_ARG;
+    if (L->partial) return MF_ERR_UNSUPPORTED;  // filled from its coarse level
     Ctx *c = L->c;
     FillArgs f = fill_args(L, L->arr[0], 1);
     f.ts = nullptr;
@@ -1695,6 +1745,7 @@ int level_steps(Level *L, int nsteps, int mode) {
 int mf_level_step(void *level, int nsteps, int mode) {
     Level *L = (Level *)level;
     if (!L || nsteps < 0 || mode < 0 || mode > 2) return MF_ERR_BAD_ARG;
+    if (L->hier) return MF_ERR_UNSUPPORTED;  // stepped by its hierarchy
     int st = level_steps(L, nsteps, mode);
     if (st) return st;
     if (nsteps > 0) L->lag_mode = 0;
@@ -1791,6 +1842,7 @@ int level_chem_phase(Level *L, double react_time, double dt_react, double dt_rk)
 int mf_level_step_timed(void *level, int nsteps, int mode, double *class_ms, int *class_launches) {
     Level *L = (Level *)level;
     if (!L || nsteps < 0 || mode < 0 || mode > 1 || !class_ms) return MF_ERR_BAD_ARG;
+    if (L->hier) return MF_ERR_UNSUPPORTED;  // stepped by its hierarchy
     Ctx *c = L->c;
     if (L->need_est) {
         enqueue_est(L);
@@ -1823,12 +1875,14 @@ int mf_level_step_timed(void *level, int nsteps, int mode, double *class_ms, int
 int mf_level_evolve(void *level, double final_time, int max_step, int only_flow, int mode, int *nsteps_out) {
     Level *L = (Level *)level;
     if (!L || mode < 0 || mode > 2 || max_step < 0) return MF_ERR_BAD_ARG;
+    if (L->hier) return MF_ERR_UNSUPPORTED;  // stepped by its hierarchy
     return level_evolve(L, final_time, max_step, only_flow ? 1 : 0, mode, nsteps_out);
 }
 
 int mf_level_advance_chemistry(void *level, double react_time, double dt_react, double dt_react_rk) {
     Level *L = (Level *)level;
     if (!L) return MF_ERR_BAD_ARG;
+    if (L->hier) return MF_ERR_UNSUPPORTED;  // stepped by its hierarchy
     return level_chem_phase(L, react_time, dt_react, dt_react_rk);
 }
 
@@ -1836,6 +1890,7 @@ int mf_level_evolve_split(void *level, double t_ss, double t_react, double dt_re
                           int max_step, int mode) {
     Level *L = (Level *)level;
     if (!L || mode < 0 || mode > 2 || max_step < 0) return MF_ERR_BAD_ARG;
+    if (L->hier) return MF_ERR_UNSUPPORTED;  // stepped by its hierarchy
     if (L->c->prm.using_bg_inertgas && !L->c->ks->has_bg_gas) return MF_ERR_UNSUPPORTED;
     double react_time = 0.0, flow_time;
     int st = level_evolve(L, t_ss, max_step, 1, mode, nullptr);
@@ -1897,6 +1952,7 @@ int mf_level_set_time(void *level, double time, double dt_prev, int istep) {
 int mf_level_set_tile(void *level, int cls, int tx, int ty, int tz) {
     Level *L = (Level *)level;
     if (!L || cls < 0 || cls >= kTileClasses) return MF_ERR_BAD_ARG;
+    if (L->hier) return MF_ERR_UNSUPPORTED;  // stepped by its hierarchy
     if (tx < 1 || ty < 1 || tz < 1 || (int64_t)tx * ty * tz > 4096) return MF_ERR_BAD_ARG;
     int t[3] = {tx, ty, tz};
     if (cls < 3 && L->c->ks->lds_bytes(t, cls) > 160 * 1024) return MF_ERR_TOO_LARGE;
@@ -1925,6 +1981,7 @@ int mf_level_get_tile(void *level, int cls, int *tx, int *ty, int *tz) {
 int mf_level_autotune(void *level) {
     Level *L = (Level *)level;
     if (!L) return MF_ERR_BAD_ARG;
+    if (L->hier) return MF_ERR_UNSUPPORTED;  // stepped by its hierarchy
     Ctx *c = L->c;
     static const int face_cand[3][10][3] = {
         {{64, 4, 1}, {64, 2, 2}, {32, 16, 1}, {32, 4, 2}, {32, 8, 1}, {64, 1, 4}, {32, 8, 2}, {32, 2, 4}, {32, 4, 4}, {32, 2, 8}},
@@ -2065,6 +2122,7 @@ int mf_util_div_check(void *ctx, int64_t n, int lo_exp, int hi_exp, int64_t *cou
 int mf_adjoint_forward(void *level, int nsteps, int ncheck, int spec, double *J_out) {
     Level *L = (Level *)level;
     if (!L || nsteps < 1 || ncheck < 1) return MF_ERR_BAD_ARG;
+    if (L->hier) return MF_ERR_UNSUPPORTED;  // stepped by its hierarchy
     Ctx *c = L->c;
     if (spec < 0 || spec >= c->ns) return MF_ERR_BAD_ARG;
     if (!c->ks->adj || !c->prm.fixed_dt || !(c->prm.dt_fixed > 0) || c->prm.do_ns != 1 || c->prm.order_hyp != 5 ||
@@ -2249,3 +2307,6 @@ int mf_ctx_fill_physbc(void *ctx, mf_array4 view, mf_box region, double time) {
     HIPCHK(hipGetLastError());
     return MF_OK;
 }
+
+// the two-level driver (mf_hier_*)
+#include "mf_hier.inc.h"

@@ -1022,6 +1022,40 @@ __global__ __launch_bounds__(256) void k_box_post(BoxPostArgs a) {
     }
 }
 
+// Advance_chemistry_implicit alone over the valid cells of a level's tile
+// list (a level of a hierarchy, whose primitive update waits for the end of
+// the coarse step): st.src -> dst (may alias), dt of st.ts
+template <class P>
+__global__ __launch_bounds__(256) void k_chem_tiles(ResArgs r, Work w) {
+    constexpr int NV = Num<P>::NV;
+    const StageArgs &a = r.st;
+    if (step_skipped(a.ts)) return;
+    const Tab &T = *a.tab;
+    const double dt = stage_dt(a);
+    for (TileIter it(w); it.ok(); it.next()) {
+        int lt;
+        const BoxDesc &b = tile_box(w, it.t, lt);
+        int c0[3], c1[3];
+        tile_range(b, w, lt, c0, c1);
+        const int ex = c1[0] - c0[0] + 1, ey = c1[1] - c0[1] + 1, ez = c1[2] - c0[2] + 1;
+        const int nc = ex * ey * ez;
+        for (int f = threadIdx.x; f < nc; f += blockDim.x) {
+            const int i = c0[0] + f % ex;
+            const int rr = f / ex;
+            const int j = c0[1] + rr % ey;
+            const int k = c0[2] + rr / ey;
+            const double *sp = a.src.p + a.src.off(i, j, k);
+            double *dp = r.dst.p + r.dst.off(i, j, k);
+            double S[NV];
+#pragma unroll
+            for (int n = 0; n < NV; ++n) S[n] = sp[(int64_t)n * a.src.ns];
+            chemistry_rk3<P>(T, S, dt);
+#pragma unroll
+            for (int n = 0; n < NV; ++n) dp[(int64_t)n * r.dst.ns] = S[n];
+        }
+    }
+}
+
 // EstTimeStep partials on the current state (first step of a run)
 template <class P>
 __global__ __launch_bounds__(256) void k_est(ResArgs r, Work w) {
@@ -1173,6 +1207,22 @@ __global__ __launch_bounds__(256) void k_fill_region(FillArgs a, int phase, int3
         iv[0] = rlo.x + (int)(r % ex); r /= ex;
         iv[1] = rlo.y + (int)(r % ey); r /= ey;
         iv[2] = rlo.z + (int)r;
+        fill_cell_rules<P>(a, iv, phase, T, time);
+    }
+}
+
+// The rules of one phase over a list of cells of a.s ([3][n]: i, j, k), the
+// ghost cells of a partly covered level that lie outside the domain
+// (FillPatchTwoLevels' physical boundary fill of the fine boxes); the sources
+// of a phase are cells of earlier phases or interior cells, so the cells of a
+// phase are independent.
+template <class P>
+__global__ __launch_bounds__(256) void k_fill_list(FillArgs a, int phase, const int *list, int n) {
+    if (step_skipped(a.ts)) return;
+    const Tab &T = *a.tab;
+    const double time = !a.ts ? a.time_host : (a.stage == 1 ? a.ts->t : a.ts->t + a.ts->dt);
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int iv[3] = {list[q], list[n + q], list[2 * n + q]};
         fill_cell_rules<P>(a, iv, phase, T, time);
     }
 }
@@ -1621,6 +1671,14 @@ struct Launch {
         hipLaunchKernelGGL(k_fill_region<P>, dim3((unsigned)grid), dim3(256), 0, st, a, phase,
                            make_int3(rlo[0], rlo[1], rlo[2]), make_int3(rhi[0], rhi[1], rhi[2]));
     }
+    static void fill_list(const FillArgs &a, int phase, const int *list, int n, hipStream_t st) {
+        if (n <= 0) return;
+        const int grid = std::min((n + 255) / 256, 65535);
+        hipLaunchKernelGGL(k_fill_list<P>, dim3((unsigned)grid), dim3(256), 0, st, a, phase, list, n);
+    }
+    static void chem(const ResArgs &r, const Work &w, int grid, int block, hipStream_t st) {
+        hipLaunchKernelGGL(k_chem_tiles<P>, dim3(grid), dim3(block), 0, st, r, w);
+    }
     static void cut(const CutArgs &a, hipStream_t st) {
         if (a.ncut <= 0) return;
         int g = (a.ncut + 127) / 128;
@@ -1695,6 +1753,8 @@ struct Launch {
         k.adj = adj;
         k.fill = &fill;
         k.fill_region = &fill_region;
+        k.fill_list = &fill_list;
+        k.chem = &chem;
         k.cut = &cut;
         k.cut_geom = &cut_geom;
         k.cut_box = &cut_box;

@@ -83,6 +83,18 @@ SIGNATURES = {
     "mf_adjoint_forward": ([C.c_void_p, C.c_int, C.c_int, C.c_int, _P(C.c_double)], C.c_int),
     "mf_adjoint_backward": ([C.c_void_p, C.c_void_p, _P(C.c_double)], C.c_int),
     "mf_adjoint_state_grad": ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),
+    "mf_hier_create": ([C.c_void_p, C.c_int, _P(C.c_void_p)], C.c_int),
+    "mf_hier_destroy": ([C.c_void_p], C.c_int),
+    "mf_hier_add_level": ([C.c_void_p, _P(abi.Box), C.c_int, C.c_void_p, C.c_int, _P(C.c_void_p)], C.c_int),
+    "mf_hier_level": ([C.c_void_p, C.c_int, _P(C.c_void_p)], C.c_int),
+    "mf_hier_nlevels": ([C.c_void_p, _P(C.c_int)], C.c_int),
+    "mf_hier_set_reflux": ([C.c_void_p, C.c_int], C.c_int),
+    "mf_hier_set_regrid": ([C.c_void_p, C.c_int, _P(C.c_double), _P(C.c_double), _P(C.c_int), C.c_int, C.c_int, C.c_int,
+                            C.c_int, C.c_double], C.c_int),
+    "mf_hier_regrid": ([C.c_void_p, _P(abi.Box), C.c_int], C.c_int),
+    "mf_hier_step": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    "mf_hier_evolve": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, _P(C.c_int)], C.c_int),
+    "mf_hier_info": ([C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_int), C.c_void_p], C.c_int),
     "mf_util_copy": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64], C.c_int),
     "mf_util_div_check": ([C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_void_p], C.c_int),
 }
@@ -349,6 +361,122 @@ class Level:
 
 
 _HIPMEMCPY = None
+
+
+def _box_array(boxes):
+    boxes = list(boxes)
+    arr = (abi.Box * max(len(boxes), 1))()
+    for i, (lo, hi) in enumerate(boxes):
+        arr[i] = abi.make_box(lo, hi)
+    return arr, len(boxes)
+
+
+class _LevelView(Level):
+    """A level owned by a hierarchy (level 1): the Level calls on its handle,
+    released with the hierarchy."""
+
+    def __init__(self, h, ctx, shape, nboxes):
+        self.h = h
+        self.ctx = ctx
+        self.shape = shape
+        self.nboxes = nboxes
+
+    def close(self):
+        self.h = None
+
+
+class Hierarchy:
+    """Two-level hierarchy (mf_hier_*) over a Level as level 0: mflo's
+    EvolveAMR with subcycling, flux registers, reflux, average_down and
+    regrid, stepped on the device."""
+
+    def __init__(self, level0, ratio=2):
+        self.L0 = level0
+        self.ratio = ratio
+        h = C.c_void_p()
+        check(lib().mf_hier_create(level0.h, int(ratio), C.byref(h)), "mf_hier_create")
+        self.h = h
+        nv = level0.shape[0]
+        self.fine_shape = (nv,) + tuple(q * ratio for q in level0.shape[1:])
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mf_hier_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_level(self, boxes, state):
+        """MakeNewLevelFromScratch: level 1 over boxes ((lo, hi) in fine
+        indices) with its initial data, dense over the fine domain (numpy or a
+        device tensor)."""
+        arr, nb = _box_array(boxes)
+        h = C.c_void_p()
+        if hasattr(state, "data_ptr"):
+            assert tuple(state.shape) == self.fine_shape and state.is_contiguous()
+            check(lib().mf_hier_add_level(self.h, arr, nb, C.c_void_p(state.data_ptr()), 1, C.byref(h)),
+                  "mf_hier_add_level")
+        else:
+            s = np.ascontiguousarray(state, dtype=np.float64)
+            assert s.shape == self.fine_shape
+            check(lib().mf_hier_add_level(self.h, arr, nb, s.ctypes.data_as(C.c_void_p), 0, C.byref(h)),
+                  "mf_hier_add_level")
+
+    def nlevels(self):
+        n = C.c_int()
+        check(lib().mf_hier_nlevels(self.h, C.byref(n)), "mf_hier_nlevels")
+        return n.value
+
+    def level(self, lev):
+        if lev == 0:
+            return self.L0
+        h = C.c_void_p()
+        check(lib().mf_hier_level(self.h, int(lev), C.byref(h)), "mf_hier_level")
+        nb = C.c_int()
+        check(lib().mf_level_boxes(h, None, 0, C.byref(nb), None, None), "mf_level_boxes")
+        return _LevelView(h, self.L0.ctx, self.fine_shape, nb.value)
+
+    def get_state(self, lev):
+        return self.level(lev).get_state()
+
+    def set_reflux(self, flag):
+        check(lib().mf_hier_set_reflux(self.h, int(bool(flag))), "mf_hier_set_reflux")
+
+    def set_regrid(self, regrid_int, refine_phi, refine_phigrad, comps, n_error_buf=1, blocking_factor=8,
+                   max_grid_size=32, grid_eff=0.7):
+        n = len(comps)
+        ph = (C.c_double * max(n, 1))(*refine_phi)
+        gr = (C.c_double * max(n, 1))(*refine_phigrad)
+        cp = (C.c_int * max(n, 1))(*comps)
+        check(lib().mf_hier_set_regrid(self.h, int(regrid_int), ph, gr, cp, n, int(n_error_buf), int(blocking_factor),
+                                       int(max_grid_size), float(grid_eff)), "mf_hier_set_regrid")
+
+    def regrid(self, boxes):
+        arr, nb = _box_array(boxes)
+        check(lib().mf_hier_regrid(self.h, arr, nb), "mf_hier_regrid")
+
+    def step(self, n=1, mode=abi.MODE_BATCHED):
+        check(lib().mf_hier_step(self.h, int(n), int(mode)), "mf_hier_step")
+
+    def evolve(self, final_time, max_step=2 ** 30, only_flow=False, mode=abi.MODE_BATCHED):
+        n = C.c_int()
+        check(lib().mf_hier_evolve(self.h, float(final_time), int(max_step), int(bool(only_flow)), int(mode),
+                                   C.byref(n)), "mf_hier_evolve")
+        return n.value
+
+    def info(self):
+        t, dt, ist = C.c_double(), C.c_double(), C.c_int()
+        res = np.zeros(self.L0.shape[0])
+        check(lib().mf_hier_info(self.h, C.byref(t), C.byref(dt), C.byref(ist), res.ctypes.data_as(C.c_void_p)),
+              "mf_hier_info")
+        return t.value, dt.value, ist.value, res
+
+    def boxes(self, lev=1):
+        return self.level(lev).boxes()
 
 
 def _hip_memcpy():

@@ -19,57 +19,10 @@ import numpy as np
 import pytest
 
 from compose_two_level import FHI, FLO, NC, R, compose_oracle_step, initial_state, periodic_case
+from hier_cases import all_periodic, cons_comps, fine_init, shear_state
 from mesoflow_amd import abi, cases
 from oracle import pyoracle as po
 from oracle.pyoracle import OracleHier, OracleLevel
-
-CONS = [0, 1, 2, 3, 4]
-
-
-def _cons(nv):
-    return CONS + list(range(abi.FLO_NVARS, nv))
-
-
-def _all_periodic(n, boxes=None):
-    c = cases.shocktube_n2(n_cell=(n, n, n), order_hyp=5, cfl=0.4)
-    for d in range(3):
-        c.geom.is_periodic[d] = 1
-        c.params.lo_bc[d] = abi.BC_PERIODIC
-        c.params.hi_bc[d] = abi.BC_PERIODIC
-    if boxes is not None:
-        c.boxes = boxes
-    return c
-
-
-def _fine_init(cstate, fboxes, ratio=R):
-    """fine data by cell_cons_interp of the coarse state over each fine box,
-    and the coarse state averaged down (MakeNewLevelFromCoarse + AverageDown)"""
-    nv, nz, ny, nx = cstate.shape
-    c = np.ascontiguousarray(cstate.copy())
-    f = np.zeros((nv, nz * ratio, ny * ratio, nx * ratio))
-    pad = np.ascontiguousarray(np.pad(c, ((0, 0), (1, 1), (1, 1), (1, 1)), mode="wrap"))
-    for lo, hi in fboxes:
-        n = [hi[d] - lo[d] + 1 for d in range(3)]
-        fab = np.zeros((nv, n[2], n[1], n[0]))
-        po.interp_cons((lo, hi), (pad, (-1, -1, -1)), (fab, lo), ratio)
-        f[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = fab
-    for lo, hi in fboxes:
-        clo = tuple(q // ratio for q in lo)
-        chi = tuple(q // ratio for q in hi)
-        po.average_down((clo, chi), (np.ascontiguousarray(f), (0, 0, 0)), (c, (0, 0, 0)), ratio)
-    return c, f
-
-
-def _shear_state(cc, amp=0.2):
-    s0 = cc.make_state()
-    n = cc.n_cell[1]
-    y = (np.arange(n) + 0.5) / n
-    v = amp * np.sin(2 * np.pi * y)[None, :, None] * np.ones(s0.shape[1:])
-    s0[abi.VELY] = v
-    s0[abi.RHOV] = s0[abi.DENS] * v
-    s0[abi.RHOE] += 0.5 * s0[abi.DENS] * v * v
-    return s0
-
 
 def test_step_equals_the_per_box_composition():
     cc = periodic_case(NC)
@@ -83,7 +36,7 @@ def test_step_equals_the_per_box_composition():
     c2.params.fixed_dt = 1
     c2.params.dt_fixed = dt
     flo, fhi = (FLO * R,) * 3, (FHI * R + R - 1,) * 3
-    cs, fs = _fine_init(s0, [(flo, fhi)])
+    cs, fs = fine_init(s0, [(flo, fhi)])
     L0 = OracleLevel(c2)
     L0.set_state(cs)
     H = OracleHier(L0, R)
@@ -91,7 +44,7 @@ def test_step_equals_the_per_box_composition():
     H.step(1)
     got_c = H.get_state(0)
     got_f = H.get_state(1)[:, flo[2]:fhi[2] + 1, flo[1]:fhi[1] + 1, flo[0]:fhi[0] + 1]
-    k = _cons(s0.shape[0])
+    k = cons_comps(s0.shape[0])
     assert np.array_equal(got_c[k], want_c[k])
     assert np.array_equal(got_f[k], want_f[k])
 
@@ -99,18 +52,18 @@ def test_step_equals_the_per_box_composition():
 def _conservation_run(reflux, nsteps=3):
     n = 16
     cboxes = cases.chop_boxes((n, n, n), 8)
-    cc = _all_periodic(n, cboxes)
-    s0 = _shear_state(cc)
+    cc = all_periodic(n, cboxes)
+    s0 = shear_state(cc)
     # three fine boxes: two sharing a face, one across the periodic x boundary
     fboxes = [((8, 8, 8), (15, 23, 19)), ((16, 8, 8), (21, 15, 19)), ((0, 20, 4), (5, 27, 11))]
-    cs, fs = _fine_init(s0, fboxes)
+    cs, fs = fine_init(s0, fboxes)
     L0 = OracleLevel(cc)
     L0.set_state(cs)
     H = OracleHier(L0, R)
     H.set_reflux(reflux)
     H.add_level(fboxes, fs)
     dv = (1.0 / n) ** 3
-    k = _cons(s0.shape[0])
+    k = cons_comps(s0.shape[0])
     m0 = cs[k].sum(axis=(1, 2, 3)) * dv
     H.step(nsteps)
     c1 = H.get_state(0)
@@ -129,10 +82,10 @@ def test_reflux_conserves_with_periodic_boundaries():
 
 def test_full_cover_equals_the_fine_level_alone():
     n = 8
-    cc = _all_periodic(n, cases.chop_boxes((n, n, n), 4))
-    s0 = _shear_state(cc)
+    cc = all_periodic(n, cases.chop_boxes((n, n, n), 4))
+    s0 = shear_state(cc)
     fboxes = [((0, 0, 0), (15, 15, 7)), ((0, 0, 8), (15, 15, 15))]
-    cs, fs = _fine_init(s0, fboxes)
+    cs, fs = fine_init(s0, fboxes)
     O = OracleLevel(cc)
     O.set_state(cs)
     dt = 0.5 * O.est_dt()
@@ -143,7 +96,7 @@ def test_full_cover_equals_the_fine_level_alone():
     H = OracleHier(L0, R)
     H.add_level(fboxes, fs)
     H.step(2)
-    cf = _all_periodic(2 * n, [((lo), (hi)) for lo, hi in fboxes])
+    cf = all_periodic(2 * n, [((lo), (hi)) for lo, hi in fboxes])
     cf.params.fixed_dt = 1
     cf.params.dt_fixed = dt / 2
     F = OracleLevel(cf)
@@ -155,16 +108,16 @@ def test_full_cover_equals_the_fine_level_alone():
     # the coarse level is the average of the fine level
     avg = got.reshape(got.shape[0], n, 2, n, 2, n, 2).sum(axis=(2, 4, 6))
     c = H.get_state(0)
-    k = _cons(c.shape[0])
+    k = cons_comps(c.shape[0])
     assert np.allclose(c[k], avg[k] / 8.0, rtol=1e-14, atol=1e-14)
 
 
 def test_regrid_keeps_data_and_conservation():
     n = 16
-    cc = _all_periodic(n, cases.chop_boxes((n, n, n), 8))
-    s0 = _shear_state(cc)
+    cc = all_periodic(n, cases.chop_boxes((n, n, n), 8))
+    s0 = shear_state(cc)
     fb0 = [((8, 8, 8), (15, 23, 19))]
-    cs, fs = _fine_init(s0, fb0)
+    cs, fs = fine_init(s0, fb0)
     L0 = OracleLevel(cc)
     L0.set_state(cs)
     H = OracleHier(L0, R)
@@ -191,7 +144,7 @@ def test_regrid_keeps_data_and_conservation():
         got = new_f[:, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1]
         assert np.array_equal(got, want), (lo, hi)
     # conservation across steps after the regrid
-    k = _cons(crse.shape[0])
+    k = cons_comps(crse.shape[0])
     dv = (1.0 / n) ** 3
     m0 = crse[k].sum(axis=(1, 2, 3)) * dv
     H.step(2)
@@ -203,10 +156,10 @@ def test_regrid_keeps_data_and_conservation():
 
 def test_regrid_without_boxes_clears_the_level():
     n = 8
-    cc = _all_periodic(n)
-    s0 = _shear_state(cc)
+    cc = all_periodic(n)
+    s0 = shear_state(cc)
     fb0 = [((4, 4, 4), (11, 11, 11))]
-    cs, fs = _fine_init(s0, fb0)
+    cs, fs = fine_init(s0, fb0)
     L0 = OracleLevel(cc)
     L0.set_state(cs)
     H = OracleHier(L0, R)
@@ -220,9 +173,9 @@ def test_regrid_without_boxes_clears_the_level():
 
 
 def test_fine_boxes_must_be_aligned_and_disjoint():
-    cc = _all_periodic(8)
+    cc = all_periodic(8)
     L0 = OracleLevel(cc)
-    L0.set_state(_shear_state(cc))
+    L0.set_state(shear_state(cc))
     H = OracleHier(L0, R)
     f = np.zeros(H.fine_shape)
     with pytest.raises(abi.MFError):
