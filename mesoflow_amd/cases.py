@@ -381,7 +381,7 @@ def grf_solid(n_cell, porosity, corr_len, seed):
     ky = np.fft.fftfreq(shp[1])[None, :, None]
     kx = np.fft.rfftfreq(shp[2])[None, None, :]
     k2 = (2 * np.pi) ** 2 * (kx ** 2 + ky ** 2 + kz ** 2)
-    f = np.fft.irfftn(np.fft.rfftn(w) * np.exp(-0.5 * k2 * corr_len ** 2), s=shp)
+    f = np.fft.irfftn(np.fft.rfftn(w) * np.exp(-0.5 * k2 * corr_len ** 2), s=shp, axes=(0, 1, 2))
     thr = np.quantile(f, porosity)
     return f > thr
 
@@ -441,6 +441,20 @@ def porous_reactor(n_cell=(256, 256, 256), nspec=5, nreact=2, geometry="spheres"
 
     bl = boxes if boxes is not None else (chop_boxes(n_cell, max_grid_size) if max_grid_size else [])
     return Case("porous_reactor", t, p, g, make_state, bl, nsteps)
+
+
+# Two-level porous reactor: a GRF solid at porosity 0.5 on a coarse level of
+# uneven boxes (edges 8..64), the fine level (ratio 2) made from the coarse
+# one where the volume fraction jumps (ErrorEst on vfrac, amr.refine_grad)
+AMR_TAGS = dict(refine_phi=[1.0e30], refine_phigrad=[0.45], comps=[abi.VFRAC])
+AMR_GRID = dict(n_error_buf=1, blocking_factor=8, max_grid_size=64, grid_eff=0.7)
+
+
+def porous_amr(n_cell=(128, 128, 128), edge=(8, 64), nspec=5, nreact=2, porosity=0.5, seed=20261207, corr_len=8.0,
+               nsteps=20):
+    boxes = random_boxes(n_cell, edge[0], edge[1], seed)
+    return porous_reactor(n_cell=n_cell, nspec=nspec, nreact=nreact, geometry="grf", porosity=porosity, seed=seed,
+                          corr_len=corr_len, boxes=boxes, nsteps=nsteps)
 
 
 # ---------------------------------------------------------------------------

@@ -862,8 +862,11 @@ int hier_fine_cutlist(Level *Lf) {
 // regrid of level 1 at `time` with new boxes: RemakeLevel (level 1 exists:
 // its data where the old boxes covered, the coarse level interpolated
 // elsewhere), MakeNewLevelFromCoarse (it does not), ClearLevel (no boxes)
-// (Source/mflo_gridwork.cpp:21-89); `time` is t_new of level 0, whose data is A
-int hier_regrid_to(Hier *H, const mf_box *boxes, int nb, double time) {
+// (Source/mflo_gridwork.cpp:21-89); `time` is t_new of level 0, whose data is
+// A. in_step: the regrid inside timeStep, after ComputeDt (a new level gets
+// this step's dt[0] / ratio); else between steps (dt[0] of the last step, or
+// 1e100 before the first)
+int hier_regrid_to(Hier *H, const mf_box *boxes, int nb, double time, bool in_step) {
     Level *Lc = H->L[0];
     hipStream_t st = Lc->c->stream;
     HIPCHK(hipStreamSynchronize(st));
@@ -918,7 +921,8 @@ int hier_regrid_to(Hier *H, const mf_box *boxes, int nb, double time) {
         rc = read_ts(Lc, &h0);
         if (!rc) {
             // t_new = time, t_old = time - 1e200 (:34-35, :60-61); a new level gets dt[0] / ratio (:203-206)
-            hipLaunchKernelGGL(k_hier_newlevel, dim3(1), dim3(1), 0, st, H->ht, Lf->ts, 1, time, h0.dt / H->ratio,
+            const double dt0 = in_step ? h0.dt : h0.dt_prev;
+            hipLaunchKernelGGL(k_hier_newlevel, dim3(1), dim3(1), 0, st, H->ht, Lf->ts, 1, time, dt0 / H->ratio,
                                had ? 0 : 1);
             hier_fill_fine(H, Lf->arr[0], Lc->arr[0], Lc->arr[0], 1, 1, time, cneed, (int)(S.cneed.size() / 3), fcf,
                            (int)(S.cf.size() / 3), false);
@@ -929,7 +933,9 @@ int hier_regrid_to(Hier *H, const mf_box *boxes, int nb, double time) {
     if (fcf) hipFree(fcf);
     if (rc) return rc;
     if ((rc = hier_fine_cutlist(Lf))) return rc;
-    Lf->need_est = false;
+    // the next ComputeDt estimates over the new boxes (inside a step the
+    // step's post-pass does it)
+    Lf->need_est = !in_step;
     return hier_build(H);
 }
 
@@ -954,7 +960,7 @@ int hier_maybe_regrid(Hier *H, int istep0, double time, bool *did) {
     if (nb > 0) rc = mf_tags_to_boxes(tags.data(), n, per, H->n_error_buf, H->blocking_factor, H->max_grid_size,
                                       H->grid_eff, H->ratio, bx.data(), nb, &nb);
     if (rc) return rc;
-    rc = hier_regrid_to(H, bx.data(), nb, time);
+    rc = hier_regrid_to(H, bx.data(), nb, time, true);
     if (rc) return rc;
     H->last_regrid_step = istep0;
     *did = true;
@@ -1145,7 +1151,7 @@ int mf_hier_regrid(void *hier, const mf_box *boxes, int nboxes) {
     TimeState h;
     int rc = read_ts(H->L[0], &h);
     if (rc) return rc;
-    return hier_regrid_to(H, boxes, nboxes, h.t);
+    return hier_regrid_to(H, boxes, nboxes, h.t, false);
 }
 
 int mf_hier_step(void *hier, int nsteps, int mode) {

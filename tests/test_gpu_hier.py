@@ -49,8 +49,11 @@ def _compare(OH, H, fboxes, where):
     assert ef.max() <= TOL, ef
     assert gis == ois
     assert abs(gt - ot) <= 1e-12 * abs(ot)
-    # residual norms of level 0 (sums of squares in another order: rounding only)
-    assert np.max(np.abs(gres - ores) / np.where(np.abs(ores) > 0, np.abs(ores), 1.0)) <= 1e-9, (ores, gres)
+    # residual norms of level 0: sums of squares of (S_new - S_old) / dt; a
+    # component that changes only by rounding (vfrac under average_down) has a
+    # norm of noise, so the bar is relative to the norm or to that noise floor
+    floor = 1e-12 * np.abs(oc).reshape(oc.shape[0], -1).max(axis=1) * np.sqrt(oc[0].size) / odt
+    assert np.all(np.abs(gres - ores) <= 1e-9 * np.abs(ores) + floor), (ores, gres)
 
 
 def _close(ctx, L0, H):
@@ -191,3 +194,87 @@ def test_level0_is_stepped_through_the_hierarchy():
     L0.step(1)                                           # a plain level again
     L0.close()
     ctx.close()
+
+
+def _tagged_boxes_gpu(L0, time=0.0, **grid):
+    tg = cases.AMR_TAGS
+    tags, _ = amr.level_tag(L0, time, tg["refine_phi"], tg["refine_phigrad"], tg["comps"])
+    return amr.tags_to_boxes(tags, grid["n_error_buf"], grid["blocking_factor"], grid["max_grid_size"],
+                             grid["grid_eff"], 2, periodic=(0, 1, 1))
+
+
+def test_grf_two_level_reactor_matches_the_oracle():
+    """the two-level porous reactor (GRF solid, uneven coarse boxes, the fine
+    level made from the coarse one where vfrac jumps, MakeNewLevelFromCoarse)
+    at 32^3, with a regrid every 2 steps, against the oracle"""
+    cc = cases.porous_amr(n_cell=(32, 32, 32), edge=(8, 16), corr_len=4.0)
+    s0 = cc.make_state()
+    grid = dict(cases.AMR_GRID, max_grid_size=16)
+    tg = cases.AMR_TAGS
+    crit = (tg["refine_phi"], tg["refine_phigrad"], tg["comps"])
+    O0 = OracleLevel(cc)
+    O0.set_state(s0)
+    OH = OracleHier(O0, 2)
+    ctx = runtime.Context(cc)
+    L0 = runtime.Level(ctx)
+    L0.set_state(s0)
+    fb = _tagged_boxes_gpu(L0, **grid)
+    ob = _oracle_regrid_boxes(cc, O0, 0.0, crit, grid["blocking_factor"], grid["max_grid_size"], grid["grid_eff"])
+    assert sorted(fb) == sorted(ob) and fb
+    H = runtime.Hierarchy(L0, 2)
+    H.regrid(fb)
+    OH.regrid(ob, 0.0)
+    H.set_regrid(2, *crit, **grid)
+    H.step(4)
+    for s in range(4):
+        t, _, ist, _ = OH.info()
+        if ist > 0 and ist % 2 == 0:
+            OH.step(1, regrid_boxes=_oracle_regrid_boxes(cc, O0, t, crit, grid["blocking_factor"],
+                                                        grid["max_grid_size"], grid["grid_eff"]))
+        else:
+            OH.step(1)
+    got_boxes = H.boxes(1)
+    say("GRF reactor 32^3: %d coarse boxes, %d fine boxes" % (len(cc.boxes), len(got_boxes)))
+    _compare(OH, H, got_boxes, "GRF two-level reactor, 4 steps")
+    _close(ctx, L0, H)
+
+
+def test_grf_two_level_reactor_full_size():
+    """the two-level reactor at its full size: 128^3 coarse level in uneven
+    boxes, the fine level from the tags (about 10.7M fine cells in about 1300
+    boxes): 6 coarse steps as persistent launches and replayed from a graph
+    give the same state bit for bit, finite, with positive pressure in the
+    fluid and non-negative species"""
+    cc = cases.porous_amr()
+    s0 = cc.make_state()
+    out = []
+    for mode in (abi.MODE_BATCHED, abi.MODE_GRAPH):
+        ctx = runtime.Context(cc)
+        L0 = runtime.Level(ctx)
+        L0.set_state(s0)
+        fb = _tagged_boxes_gpu(L0, **cases.AMR_GRID)
+        H = runtime.Hierarchy(L0, 2)
+        H.regrid(fb)
+        ncf = sum(np.prod([h[d] - l[d] + 1 for d in range(3)]) for l, h in fb)
+        H.step(1, mode)
+        ctx.sync()
+        import time
+        t0 = time.time()
+        H.step(5, mode)
+        ctx.sync()
+        el = time.time() - t0
+        cu = 5 * (128 ** 3 + 2 * ncf) / el
+        say("two-level reactor 128^3: %d coarse + %d fine boxes, %.2fM fine cells, mode %d: %.1f ms per coarse step, "
+            "%.3g cell-updates/s" % (len(cc.boxes), len(fb), ncf / 1e6, mode, 1e3 * el / 5, cu))
+        c = H.get_state(0)
+        f = H.get_state(1)
+        m = fine_mask(f.shape, fb)
+        out.append((c, f[:, m]))
+        _close(ctx, L0, H)
+    (c0, f0), (c1, f1) = out
+    assert np.array_equal(c0, c1) and np.array_equal(f0, f1)
+    for a in (c0, f0):
+        assert np.all(np.isfinite(a))
+        fl = a[abi.VFRAC] == 1.0
+        assert np.all(a[abi.PRES][fl] > 0)
+        assert np.all(a[abi.FLO_NVARS:] >= 0)
