@@ -62,21 +62,40 @@ def nv_of(case):
     return 12 + case.nspec
 
 
+def host_cpu():
+    """CPU model, logical CPUs of the host and those this process may use"""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
+
+
 def cpu_baseline(name, budget_s):
-    """Oracle (CPU restatement, OpenMP) on a bounded sample of the workload."""
+    """Oracle (CPU restatement, OpenMP over every CPU this process may use,
+    capped by OMP_NUM_THREADS when set) on a bounded sample of the workload."""
     from mesoflow_amd import cases
     from oracle.pyoracle import OracleLevel
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model, nproc, usable = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS", usable))
     if name == "sod64":
         c = cases.shocktube_n2(n_cell=(64, 64, 64), prob_hi=(1.0, 1.0, 1.0), order_hyp=5, cfl=0.5)
         sample = "Sod 64^3, WENO5"
     else:
-        n = 64
+        n = 128
         c = cases.porous_reactor(n_cell=(n, n, n), nspec=5 if name != "porous512" else 10,
                                  nreact=2 if name != "porous512" else 8,
                                  geometry="spheres" if name != "porous512" else "grf", porosity=0.6 if name != "porous512" else 0.45,
                                  rmin=6.0, rmax=14.0, corr_len=8.0, order_hyp=5, cfl=0.3, seed=20261203)
-        sample = "same physics on a 64^3 sub-problem (sphere pack, NS=%d)" % c.nspec
+        sample = "same physics on a 128^3 sub-problem (1/8 of the cells; sphere pack, NS=%d)" % c.nspec
     L = OracleLevel(c)
     L.set_state(c.make_state())
     cells = float(np.prod(c.n_cell))
@@ -89,8 +108,10 @@ def cpu_baseline(name, budget_s):
     L.step(nmore)
     t3 = time.time()
     rate = cells * nmore / (t3 - t2)
-    return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": "%s, %d RK3 steps after 1 warm-up step, %.1f s, OMP threads %d" % (sample, nmore, t3 - t2, threads)}
+    return {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port", "cpu_model": model,
+            "nproc": nproc, "cpus_usable": usable,
+            "sample": "%s, %d RK3 steps after 1 warm-up step, %.1f s, OMP threads %d of %d usable (%d on the host)"
+                      % (sample, nmore, t3 - t2, threads, usable, nproc)}
 
 
 def adjoint_bench(nsteps=200, ncheck_binomial=20):
@@ -183,13 +204,96 @@ def split_chem_cpu(name, nsub=100, dt_rk=1.0e-6):
 
 
 def read_traffic(config):
+    """(HBM bytes per face-sweep launch, the profile's tile line) from the
+    committed rocprofv3 --pmc summary of this workload"""
     path = os.path.join(ROOT, "profiles", "traffic_%s.txt" % config)
     if not os.path.exists(path):
-        return None
+        return None, None
+    val, tile = None, None
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1])
-    return None
+            val = float(line.split(":")[1])
+        if line.startswith("tile:"):
+            tile = line.split(":", 1)[1].strip()
+    return val, tile
+
+
+def box_rhs_bench(ctx, lvl, reps, edge=32):
+    """The drop-in per-box call (mf_box_rhs with face-flux output) on one
+    edge^3 box of the level's ghost-filled state: ms per call and cell-RHS
+    evaluations per second"""
+    import torch
+    from mesoflow_amd import abi, runtime
+    nv = ctx.nv
+    pv = abi.Array4()
+    runtime.check(runtime.lib().mf_level_state_view(lvl.h, pv), "mf_level_state_view")
+    lvl.fill_ghosts(0.0)
+    ctx.sync()
+    lo = (0, 0, 0)
+    hi = (edge - 1,) * 3
+    ds = torch.zeros((nv, edge, edge, edge), dtype=torch.float64, device="cuda")
+    fl, fv = [], (abi.Array4 * 3)()
+    for d in range(3):
+        m = [edge] * 3
+        m[d] += 1
+        t = torch.zeros((nv, m[2], m[1], m[0]), dtype=torch.float64, device="cuda")
+        fl.append(t)
+        fv[d] = abi.make_array4(t.data_ptr(), lo, [lo[q] + m[q] for q in range(3)], nv)
+    dsv = abi.make_array4(ds.data_ptr(), lo, [edge] * 3, nv)
+    torch.cuda.synchronize()
+
+    def call():
+        runtime.check(runtime.lib().mf_box_rhs(ctx.h, abi.make_box(lo, hi), pv, dsv, fv, 0.0), "mf_box_rhs")
+
+    call()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    ctx.sync()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    return {"box": "%d^3 of the workload's state, flux views given, all components" % edge, "reps": reps,
+            "ms_per_call": round(ms, 4), "cell_rhs_per_s": edge ** 3 / (ms * 1e-3)}
+
+
+def amr_two_level_bench(steps):
+    """The two-level reactor: 128^3 coarse level in uneven boxes (edges 8..64),
+    GRF solid at porosity 0.5, 5 species, the fine level (ratio 2,
+    subcycled) made from the coarse one where vfrac jumps; coarse steps of the
+    hierarchy (mf_hier_step) timed after one untimed step"""
+    import torch
+    from mesoflow_amd import abi, amr, cases, runtime
+    c = cases.porous_amr()
+    s0 = c.make_state()
+    ctx = runtime.Context(c)
+    L0 = runtime.Level(ctx)
+    L0.set_state(s0)
+    tg, gr = cases.AMR_TAGS, cases.AMR_GRID
+    tags, _ = amr.level_tag(L0, 0.0, tg["refine_phi"], tg["refine_phigrad"], tg["comps"])
+    fb = amr.tags_to_boxes(tags, gr["n_error_buf"], gr["blocking_factor"], gr["max_grid_size"], gr["grid_eff"], 2,
+                           periodic=(0, 1, 1))
+    H = runtime.Hierarchy(L0, 2)
+    H.regrid(fb)
+    nfine = int(sum(np.prod([h[d] - l[d] + 1 for d in range(3)]) for l, h in fb))
+    ncrse = int(np.prod(c.n_cell))
+    out = {"workload": "two-level porous reactor: 128^3 coarse level in %d uneven boxes (edges 8..64), GRF IB porosity "
+                       "0.5, 5 species, 2 reactions, WENO5; fine level ratio 2 subcycled, %d boxes, %d cells "
+                       "(%.1f%% of 256^3), from the vfrac tags" % (len(c.boxes), len(fb), nfine, 100.0 * nfine / 256 ** 3),
+           "coarse_boxes": len(c.boxes), "fine_boxes": len(fb), "fine_cells": nfine, "steps": steps}
+    for name, m in (("batched", abi.MODE_BATCHED), ("graph", abi.MODE_GRAPH)):
+        H.step(1, m)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        H.step(steps, m)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / steps
+        out[name] = {"ms_per_coarse_step": el * 1e3, "cell_updates_per_s": (ncrse + 2 * nfine) / el}
+        log("two-level %s: %.2f ms per coarse step" % (name, el * 1e3))
+    H.close()
+    L0.close()
+    ctx.close()
+    torch.cuda.empty_cache()
+    return out
 
 
 def amr_ops_bench(ctx, lvl, reps):
@@ -274,6 +378,10 @@ def main():
                     help="steps of the discrete-adjoint measurement at 128^3 (0 = skip)")
     ap.add_argument("--amr-reps", type=int, default=5,
                     help="launches of each two-level coupling operator and of the tagging at full size (0 = skip)")
+    ap.add_argument("--amr-steps", type=int, default=5,
+                    help="coarse steps of the two-level reactor (128^3 coarse, subcycled fine level; 0 = skip)")
+    ap.add_argument("--box-rhs-reps", type=int, default=20,
+                    help="calls of the per-box mf_box_rhs on a 32^3 box (0 = skip)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -346,10 +454,13 @@ def main():
         avg_s = ms_faces / max(n_faces, 1) * 1e-3
         alg_bytes = step_bytes / 9.0
         achieved = alg_bytes / avg_s / 1e9
-        traffic = read_traffic(args.config)
+        traffic, traffic_tile = read_traffic(args.config)
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_faces (WENO5 face-flux sweep)",
-                "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes_per_launch": alg_bytes}
+                "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes_per_launch": alg_bytes,
+                "traffic_source": ("rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE of profiles/traffic_%s.txt (tiles %s), "
+                                   "scaled by the plain-copy calibration; not measured in this run"
+                                   % (args.config, traffic_tile)) if traffic else None}
         tot = sum(v[0] for v in classes.values())
         extra["kernel_time_share"] = {k: round(v[0] / tot, 4) for k, v in classes.items()}
     extra["step_hbm_frac"] = (step_bytes * args.steps / secs / 1e9) / HBM_PEAK_GBS
@@ -394,15 +505,23 @@ def main():
         extra["split_chemistry"] = sc
         log("split chemistry: %s" % json.dumps(sc))
 
+    if args.box_rhs_reps > 0:
+        extra["box_rhs"] = box_rhs_bench(ctx, lvl, args.box_rhs_reps)
+        log("mf_box_rhs: %s" % json.dumps(extra["box_rhs"]))
+
     if args.amr_reps > 0:
         extra["amr_ops"] = amr_ops_bench(ctx, lvl, args.amr_reps)
         log("amr operators: %s" % json.dumps(extra["amr_ops"]))
 
-    if args.adjoint_steps > 0:
+    if args.adjoint_steps > 0 or args.amr_steps > 0:
         lvl.close()
         ctx.close()
         lvl = ctx = None
         torch.cuda.empty_cache()
+    if args.amr_steps > 0:
+        log("two-level reactor ...")
+        extra["two_level"] = amr_two_level_bench(args.amr_steps)
+    if args.adjoint_steps > 0:
         log("discrete adjoint at 128^3 ...")
         extra["adjoint"] = adjoint_bench(args.adjoint_steps)
 
