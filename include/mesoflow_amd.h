@@ -444,6 +444,17 @@ int mf_hier_step(void *hier, int nsteps, int mode);
 int mf_hier_evolve(void *hier, double final_time, int max_step, int only_flow, int mode, int *nsteps_out);
 /* Time, dt[0], coarse step count and residual norms (level 0). */
 int mf_hier_info(void *hier, double *time, double *dt, int *istep, double *resnorms);
+/* Output of the hierarchy (Source/mflo_plot.cpp): WritePlotFile with
+ * amrex::WriteMultiLevelPlotfile (:45-56; Header with finest_level, the
+ * refinement ratio, each level's domain, steps and cell sizes, and
+ * Level_0/Cell, Level_1/Cell), WriteCheckpointFile (:57-145; finest_level,
+ * istep, dt and t_new of both levels, each level's BoxArray and
+ * Level_<l>/<varnames[0]>) and ReadCheckpointFile (:147-239) into a hierarchy
+ * over a level 0 with the same boxes (level 1 is made or remade with the
+ * file's boxes, or cleared). varnames as mf_level_write_plotfile. */
+int mf_hier_write_plotfile(void *hier, const char *dir, const char *const *varnames);
+int mf_hier_write_checkpoint(void *hier, const char *dir, const char *const *varnames);
+int mf_hier_read_checkpoint(void *hier, const char *dir, const char *const *varnames);
 
 /* Output of a level (Source/mflo_plot.cpp). varnames: nv names
  * (Source/mflo.cpp:34-53 order), or NULL for rho, rhou, rhov, rhow, rhoe,
@@ -458,8 +469,9 @@ int mf_level_write_plotfile(void *level, const char *dir, const char *const *var
 int mf_level_write_checkpoint(void *level, const char *dir, const char *const *varnames);
 /* ReadCheckpointFile (:147-239) into a level created with the same boxes
  * (MF_ERR_BAD_ARG if the Header's BoxArray or the FABs differ,
- * MF_ERR_UNSUPPORTED for finest_level > 0): the state (mf_level_set_state)
- * and the time, dt and step (mf_level_set_time). */
+ * MF_ERR_UNSUPPORTED for finest_level > 0: mf_hier_read_checkpoint reads
+ * those): the state (mf_level_set_state) and the time, dt and step
+ * (mf_level_set_time). */
 int mf_level_read_checkpoint(void *level, const char *dir, const char *const *varnames);
 
 /* The same files from host data (no device work): state is

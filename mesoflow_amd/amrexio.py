@@ -1,4 +1,4 @@
-"""Plotfiles and checkpoints of a level (Source/mflo_plot.cpp).
+"""Plotfiles and checkpoints of a level or a two-level hierarchy (Source/mflo_plot.cpp).
 
 Writing goes through the C-ABI (mf_write_plotfile, mf_write_checkpoint,
 mf_read_checkpoint and their mf_level_* forms); this module binds the host
@@ -164,8 +164,9 @@ def read_vismf(prefix, shape, fill=np.nan):
 
 
 def read_plotfile(path, fill=np.nan):
-    """a one-level plotfile: dict of the Header's fields and the data
-    [nv][nz][ny][nx] (fill outside the boxes)"""
+    """a plotfile of one or more levels: dict of the Header's fields, the data
+    of level 0 [nv][nz][ny][nx] (fill outside the boxes) and "levels", one
+    dict per level (domain, level steps, cell sizes, grids, boxes, data)"""
     lines = open(os.path.join(path, "Header")).read().split("\n")
     if lines[0] != "HyperCLaw-V1.1":
         raise ValueError("not a HyperCLaw-V1.1 plotfile")
@@ -175,29 +176,66 @@ def read_plotfile(path, fill=np.nan):
     dim = int(lines[i])
     time = float(lines[i + 1])
     finest = int(lines[i + 2])
-    if dim != 3 or finest != 0:
-        raise ValueError("only 3-D single-level plotfiles are read here")
+    if dim != 3:
+        raise ValueError("only 3-D plotfiles are read here")
+    nl = finest + 1
     prob_lo = [float(x) for x in lines[i + 3].split()]
     prob_hi = [float(x) for x in lines[i + 4].split()]
-    ref = lines[i + 5].split()
-    dom = _parse_box(lines[i + 6])
-    steps = int(lines[i + 7])
-    dx = [float(x) for x in lines[i + 8].split()]
-    coord = int(lines[i + 9])
-    bwidth = int(lines[i + 10])
-    lev, ngrid, ltime = lines[i + 11].split()
-    lsteps = int(lines[i + 12])
-    grids = []
-    for q in range(int(ngrid)):
-        b = [tuple(float(x) for x in lines[i + 13 + 3 * q + d].split()) for d in range(3)]
-        grids.append(b)
-    mfpath = lines[i + 13 + 3 * int(ngrid)]
-    n = [dom[1][q] - dom[0][q] + 1 for q in range(3)]
-    data, boxes, mn, mx = read_vismf(os.path.join(path, mfpath), (nv, n[2], n[1], n[0]), fill)
-    return {"names": names, "time": time, "prob_lo": prob_lo, "prob_hi": prob_hi, "ref_ratio": ref,
-            "domain": dom, "istep": steps, "dx": dx, "coord_sys": coord, "bwidth": bwidth, "level": int(lev),
-            "level_time": float(ltime), "level_steps": lsteps, "grids": grids, "boxes": boxes, "min": mn,
-            "max": mx, "data": data}
+    ref = [int(x) for x in lines[i + 5].split()]
+    doms = re.findall(r"\(\(-?\d+,-?\d+,-?\d+\) \(-?\d+,-?\d+,-?\d+\) \(\d+,\d+,\d+\)\)", lines[i + 6])
+    doms = [_parse_box(d) for d in doms]
+    steps = [int(x) for x in lines[i + 7].split()]
+    dxs = [[float(x) for x in lines[i + 8 + l].split()] for l in range(nl)]
+    j = i + 8 + nl
+    coord = int(lines[j])
+    bwidth = int(lines[j + 1])
+    j += 2
+    levels = []
+    for l in range(nl):
+        lev, ngrid, ltime = lines[j].split()
+        lsteps = int(lines[j + 1])
+        grids = []
+        for q in range(int(ngrid)):
+            grids.append([tuple(float(x) for x in lines[j + 2 + 3 * q + d].split()) for d in range(3)])
+        mfpath = lines[j + 2 + 3 * int(ngrid)]
+        j += 3 + 3 * int(ngrid)
+        dom = doms[l]
+        n = [dom[1][q] - dom[0][q] + 1 for q in range(3)]
+        data, boxes, mn, mx = read_vismf(os.path.join(path, mfpath), (nv, n[2], n[1], n[0]), fill)
+        levels.append({"level": int(lev), "level_time": float(ltime), "level_steps": lsteps, "domain": dom,
+                       "istep": steps[l], "dx": dxs[l], "grids": grids, "boxes": boxes, "min": mn, "max": mx,
+                       "data": data})
+    L0 = levels[0]
+    return {"names": names, "time": time, "finest_level": finest, "prob_lo": prob_lo, "prob_hi": prob_hi,
+            "ref_ratio": ref, "domain": L0["domain"], "istep": L0["istep"], "dx": L0["dx"], "coord_sys": coord,
+            "bwidth": bwidth, "level": 0, "level_time": L0["level_time"], "level_steps": L0["level_steps"],
+            "grids": L0["grids"], "boxes": L0["boxes"], "min": L0["min"], "max": L0["max"], "data": L0["data"],
+            "levels": levels}
+
+
+def read_checkpoint_levels(path, nv, n_cells, varnames=None, fill=np.nan):
+    """a checkpoint of one or more levels: dict(finest_level, istep, dt, time
+    (one entry per possible level), boxes and data per existing level);
+    n_cells: the domain of each level"""
+    lines = open(os.path.join(path, "Header")).read().split("\n")
+    if lines[0] != "Checkpoint file for mflo":
+        raise ValueError("not an mflo checkpoint")
+    finest = int(lines[1])
+    istep = [int(x) for x in lines[2].split()]
+    dt = [float(x) for x in lines[3].split()]
+    time = [float(x) for x in lines[4].split()]
+    i = 5
+    v0 = varnames[0] if varnames else "rho"
+    boxes, data = [], []
+    for l in range(finest + 1):
+        b, i = _read_boxarray(lines, i)
+        nx, ny, nz = n_cells[l]
+        d, fboxes, _, _ = read_vismf(os.path.join(path, "Level_%d" % l, v0), (nv, nz, ny, nx), fill)
+        if fboxes != b:
+            raise ValueError("the MultiFab's boxes differ from the Header's")
+        boxes.append(b)
+        data.append(d)
+    return {"finest_level": finest, "istep": istep, "dt": dt, "time": time, "boxes": boxes, "data": data}
 
 
 def read_checkpoint(path, nv, n_cell, varnames=None, fill=np.nan):

@@ -35,6 +35,8 @@
 // the passes), which a hipGraph replays. Every product that feeds an add in
 // the coupling arithmetic is rounded on its own, as on the CPU.
 
+#include "mf_io.h"
+
 namespace {
 
 constexpr int kHierMaxLev = 2;
@@ -139,6 +141,7 @@ __global__ void k_hier_begin(HierTime *ht, TimeState *ts_l, const TimeState *ts0
     ht->t_old[lev] = ht->t_new[lev];
     ht->t_new[lev] = ht->t_new[lev] + ht->dt[lev];
     ts_l->t = time;
+    if (lev > 0) ts_l->istep += 1;  // ++istep[lev] of each substep (level 0: k_finalize)
 }
 
 __global__ void k_hier_zero(double *p, int64_t n, const TimeState *ts0) {
@@ -1185,6 +1188,117 @@ int mf_hier_info(void *hier, double *time, double *dt, int *istep, double *resno
     Hier *H = (Hier *)hier;
     if (!H) return MF_ERR_BAD_ARG;
     return mf_level_info(H->L[0], time, dt, istep, resnorms);
+}
+
+}  // extern "C"
+
+namespace {
+// a level's part of the output files (its state copied to the host)
+int hier_level_data(Hier *H, int l, std::vector<double> &buf, mfio::LevelData &D) {
+    Level *L = H->L[l];
+    D.geom = L->c->geom;
+    D.boxes = L->boxes;
+    D.nv = L->c->nv;
+    buf.resize((size_t)D.nv * L->n[0] * L->n[1] * L->n[2]);
+    int rc = mf_level_get_state(L, buf.data(), 0);
+    D.state = buf.data();
+    return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int mf_hier_write_plotfile(void *hier, const char *dir, const char *const *varnames) {
+    Hier *H = (Hier *)hier;
+    if (!H || !dir) return MF_ERR_BAD_ARG;
+    std::vector<mfio::LevelData> D((size_t)H->nlev);
+    std::vector<std::vector<double>> buf((size_t)H->nlev);
+    std::vector<int> isteps;
+    double t = 0.0;
+    for (int l = 0; l < H->nlev; ++l) {
+        TimeState h;
+        int rc = read_ts(H->L[l], &h);
+        if (!rc) rc = hier_level_data(H, l, buf[(size_t)l], D[(size_t)l]);
+        if (rc) return rc;
+        isteps.push_back(h.istep);
+        if (l == 0) t = h.t;
+    }
+    return mfio::write_plotfile_levels(D, t, isteps, H->ratio, dir, varnames);
+}
+
+int mf_hier_write_checkpoint(void *hier, const char *dir, const char *const *varnames) {
+    Hier *H = (Hier *)hier;
+    if (!H || !dir) return MF_ERR_BAD_ARG;
+    std::vector<mfio::LevelData> D((size_t)H->nlev);
+    std::vector<std::vector<double>> buf((size_t)H->nlev);
+    std::vector<int> isteps(kHierMaxLev, 0);
+    std::vector<double> dt(kHierMaxLev, 0.0), t_new(kHierMaxLev, 0.0);
+    for (int l = 0; l < kHierMaxLev; ++l) {
+        TimeState h;
+        int rc = read_ts(H->L[l < H->nlev ? l : 0], &h);
+        if (rc) return rc;
+        isteps[(size_t)l] = l < H->nlev ? h.istep : 0;
+        dt[(size_t)l] = l < H->nlev ? h.dt_prev : h.dt_prev / H->ratio;
+        t_new[(size_t)l] = h.t;
+        if (l < H->nlev && (rc = hier_level_data(H, l, buf[(size_t)l], D[(size_t)l]))) return rc;
+    }
+    return mfio::write_checkpoint_levels(D, t_new, dt, isteps, dir, varnames);
+}
+
+int mf_hier_read_checkpoint(void *hier, const char *dir, const char *const *varnames) {
+    Hier *H = (Hier *)hier;
+    if (!H || !dir) return MF_ERR_BAD_ARG;
+    int finest = -1;
+    std::vector<int> isteps;
+    std::vector<double> dt, t_new;
+    std::vector<std::vector<mf_box>> bx;
+    int rc = mfio::read_checkpoint_header(dir, kHierMaxLev - 1, &finest, isteps, dt, t_new, bx);
+    if (rc) return rc;
+    Level *L0 = H->L[0];
+    if (bx[0].size() != L0->boxes.size() ||
+        std::memcmp(bx[0].data(), L0->boxes.data(), sizeof(mf_box) * bx[0].size()) != 0)
+        return MF_ERR_BAD_ARG;
+    if (finest == 1 && (rc = hier_check_boxes(H, bx[1].data(), (int)bx[1].size()))) return rc;
+    HIPCHK(hipStreamSynchronize(L0->c->stream));
+    hier_drop_graph(H);
+    {
+        std::vector<double> s0((size_t)L0->c->nv * L0->n[0] * L0->n[1] * L0->n[2]);
+        if ((rc = mf_level_get_state(L0, s0.data(), 0))) return rc;
+        if ((rc = mfio::read_checkpoint_level(dir, 0, L0->c->geom, bx[0], L0->c->nv, s0.data(), varnames))) return rc;
+        if ((rc = mf_level_set_state(L0, s0.data(), 0))) return rc;
+        if ((rc = mf_level_set_time(L0, t_new[0], dt[0], isteps[0]))) return rc;
+    }
+    const bool same = finest == 1 && H->nlev > 1 && H->L[1]->boxes.size() == bx[1].size() &&
+                      std::memcmp(H->L[1]->boxes.data(), bx[1].data(), sizeof(mf_box) * bx[1].size()) == 0;
+    if (H->nlev > 1 && !same) {
+        H->L[1]->hier = nullptr;
+        level_free(H->L[1]);
+        H->L[1] = nullptr;
+        H->nlev = 1;
+    }
+    HierTime t;
+    std::memset(&t, 0, sizeof(t));
+    for (int l = 0; l < kHierMaxLev; ++l) {
+        t.t_new[l] = t_new[(size_t)std::min(l, (int)t_new.size() - 1)];
+        t.t_old[l] = -1.e100;
+        t.dt[l] = dt[(size_t)std::min(l, (int)dt.size() - 1)];
+    }
+    if (finest == 1) {
+        if (H->nlev < 2) {
+            Level *Lf = nullptr;
+            if ((rc = hier_new_fine(H, bx[1].data(), (int)bx[1].size(), &Lf))) return rc;
+            H->L[1] = Lf;
+            H->nlev = 2;
+        }
+        Level *Lf = H->L[1];
+        std::vector<double> s1((size_t)Lf->c->nv * Lf->n[0] * Lf->n[1] * Lf->n[2], 0.0);
+        if ((rc = mfio::read_checkpoint_level(dir, 1, Lf->c->geom, bx[1], Lf->c->nv, s1.data(), varnames))) return rc;
+        if ((rc = mf_level_set_state(Lf, s1.data(), 0))) return rc;
+        if ((rc = mf_level_set_time(Lf, t_new[1], dt[1], isteps[1]))) return rc;
+    }
+    HIPCHK(hipMemcpy(H->ht, &t, sizeof(t), hipMemcpyHostToDevice));
+    H->last_regrid_step = 0;  // the static last_regrid_step of a fresh run (:185)
+    return hier_build(H);
 }
 
 }  // extern "C"

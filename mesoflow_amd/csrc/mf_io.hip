@@ -21,6 +21,7 @@
 // check.
 #include <sys/stat.h>
 
+#include <array>
 #include <cerrno>
 #include <cfloat>
 #include <cstdio>
@@ -33,6 +34,7 @@
 #include <vector>
 
 #include "mesoflow_amd.h"
+#include "mf_io.h"
 
 int mf_level_dt_prev(void *level, double *dt_prev);  // mf_capi.hip
 
@@ -225,79 +227,194 @@ int host_info(const mf_geom *geom, const mf_box *boxes, int nboxes, int nv, Leve
     return MF_OK;
 }
 
-int write_plotfile(const LevelInfo &L, const double *state, double t, int istep, const char *dir,
-                   const char *const *varnames) {
+}  // namespace
+
+namespace mfio {
+
+// amrex::WriteMultiLevelPlotfile (Source/mflo_plot.cpp:45-56): the Header of
+// WriteGenericPlotfileHeader and one Level_<l>/Cell MultiFab per level; time
+// is t_new of level 0 for every level, isteps the levels' step counts
+namespace {
+LevelInfo info_of(const LevelData &D) {
+    LevelInfo L;
+    L.geom = D.geom;
+    L.nv = D.nv;
+    L.boxes = D.boxes;
+    for (int d = 0; d < 3; ++d) L.n[d] = D.geom.n_cell[d];
+    return L;
+}
+}  // namespace
+
+int write_plotfile_levels(const std::vector<LevelData> &levels, double t, const std::vector<int> &isteps, int ratio,
+                          const char *dir, const char *const *varnames) {
     const std::string d(dir);
-    if (!make_dir(d) || !make_dir(d + "/Level_0")) return MF_ERR_BAD_ARG;
-    if (!write_vismf(d + "/Level_0/Cell", "Cell", L, state)) return MF_ERR_BAD_ARG;
+    const int nl = (int)levels.size();
+    if (nl < 1) return MF_ERR_BAD_ARG;
+    std::vector<LevelInfo> Ls;
+    std::vector<const double *> states;
+    for (const LevelData &D : levels) { Ls.push_back(info_of(D)); states.push_back(D.state); }
+    if (!make_dir(d)) return MF_ERR_BAD_ARG;
+    for (int l = 0; l < nl; ++l) {
+        const std::string ld = d + "/Level_" + std::to_string(l);
+        if (!make_dir(ld) || !write_vismf(ld + "/Cell", "Cell", Ls[l], states[l])) return MF_ERR_BAD_ARG;
+    }
     std::ofstream h(d + "/Header", std::ios::trunc);
     if (!h) return MF_ERR_BAD_ARG;
     h << std::setprecision(17);
-    const mf_geom &g = L.geom;
-    double dx[3];
-    for (int q = 0; q < 3; ++q) dx[q] = (g.prob_hi[q] - g.prob_lo[q]) / g.n_cell[q];
-    // amrex::WriteGenericPlotfileHeader for one level
-    h << "HyperCLaw-V1.1\n" << L.nv << '\n';
-    for (int c = 0; c < L.nv; ++c) h << var_name(varnames, c) << '\n';
-    h << 3 << '\n' << t << '\n' << 0 << '\n';
+    const LevelInfo &L0 = Ls[0];
+    const mf_geom &g = L0.geom;
+    h << "HyperCLaw-V1.1\n" << L0.nv << '\n';
+    for (int c = 0; c < L0.nv; ++c) h << var_name(varnames, c) << '\n';
+    h << 3 << '\n' << t << '\n' << nl - 1 << '\n';
     for (int q = 0; q < 3; ++q) h << g.prob_lo[q] << ' ';
     h << '\n';
     for (int q = 0; q < 3; ++q) h << g.prob_hi[q] << ' ';
     h << '\n';
-    h << '\n';  // refinement ratios: none below the finest level
-    mf_box dom;
-    for (int q = 0; q < 3; ++q) { dom.lo[q] = 0; dom.hi[q] = g.n_cell[q] - 1; }
-    h << box_str(dom) << ' ' << '\n' << istep << ' ' << '\n';
-    for (int q = 0; q < 3; ++q) h << dx[q] << ' ';
+    for (int l = 0; l < nl - 1; ++l) h << ratio << ' ';  // refinement ratio below each level
     h << '\n';
+    for (int l = 0; l < nl; ++l) {
+        mf_box dom;
+        for (int q = 0; q < 3; ++q) { dom.lo[q] = 0; dom.hi[q] = Ls[l].geom.n_cell[q] - 1; }
+        h << box_str(dom) << ' ';
+    }
+    h << '\n';
+    for (int l = 0; l < nl; ++l) h << isteps[l] << ' ';
+    h << '\n';
+    std::vector<std::array<double, 3>> dx(nl);
+    for (int l = 0; l < nl; ++l) {
+        const mf_geom &gl = Ls[l].geom;
+        for (int q = 0; q < 3; ++q) {
+            dx[l][q] = (gl.prob_hi[q] - gl.prob_lo[q]) / gl.n_cell[q];
+            h << dx[l][q] << ' ';
+        }
+        h << '\n';
+    }
     h << 0 << '\n' << 0 << '\n';  // coordinate system (Cartesian), boundary width
-    h << 0 << ' ' << L.boxes.size() << ' ' << t << '\n' << istep << '\n';
-    for (const mf_box &b : L.boxes)
-        for (int q = 0; q < 3; ++q)
-            h << g.prob_lo[q] + b.lo[q] * dx[q] << ' ' << g.prob_lo[q] + (b.hi[q] + 1) * dx[q] << '\n';
-    h << "Level_0/Cell\n";
+    for (int l = 0; l < nl; ++l) {
+        h << l << ' ' << Ls[l].boxes.size() << ' ' << t << '\n' << isteps[l] << '\n';
+        for (const mf_box &b : Ls[l].boxes)
+            for (int q = 0; q < 3; ++q)
+                h << g.prob_lo[q] + b.lo[q] * dx[l][q] << ' ' << g.prob_lo[q] + (b.hi[q] + 1) * dx[l][q] << '\n';
+        h << "Level_" << l << "/Cell\n";
+    }
     return h ? MF_OK : MF_ERR_BAD_ARG;
+}
+
+// WriteCheckpointFile (Source/mflo_plot.cpp:57-145): finest_level, istep, dt
+// and t_new of every possible level (nlevs_max entries each), the BoxArray and
+// the state (Level_<l>/<var0>) of every existing level
+int write_checkpoint_levels(const std::vector<LevelData> &levels, const std::vector<double> &t_new,
+                            const std::vector<double> &dt, const std::vector<int> &isteps, const char *dir,
+                            const char *const *varnames) {
+    const std::string d(dir);
+    const int nl = (int)levels.size();
+    if (nl < 1) return MF_ERR_BAD_ARG;
+    std::vector<LevelInfo> Ls;
+    std::vector<const double *> states;
+    for (const LevelData &D : levels) { Ls.push_back(info_of(D)); states.push_back(D.state); }
+    if (!make_dir(d)) return MF_ERR_BAD_ARG;
+    const std::string v0 = var_name(varnames, 0);
+    for (int l = 0; l < nl; ++l) {
+        const std::string ld = d + "/Level_" + std::to_string(l);
+        if (!make_dir(ld) || !write_vismf(ld + "/" + v0, v0, Ls[l], states[l])) return MF_ERR_BAD_ARG;
+    }
+    std::ofstream h(d + "/Header", std::ios::trunc | std::ios::binary);
+    if (!h) return MF_ERR_BAD_ARG;
+    h << std::setprecision(17);
+    h << "Checkpoint file for mflo\n" << nl - 1 << '\n';
+    for (int v : isteps) h << v << ' ';
+    h << '\n';
+    for (double v : dt) h << v << ' ';
+    h << '\n';
+    for (double v : t_new) h << v << ' ';
+    h << '\n';
+    for (int l = 0; l < nl; ++l) {
+        write_boxarray(h, Ls[l].boxes);
+        h << '\n';
+    }
+    return h ? MF_OK : MF_ERR_BAD_ARG;
+}
+
+// the Header of ReadCheckpointFile (:147-239): finest_level, the arrays and
+// the BoxArray of each level
+int read_checkpoint_header(const char *dir, int max_finest, int *finest, std::vector<int> &isteps,
+                           std::vector<double> &dt, std::vector<double> &t_new, std::vector<std::vector<mf_box>> &boxes) {
+    const std::string d(dir);
+    std::ifstream h(d + "/Header");
+    if (!h) return MF_ERR_BAD_ARG;
+    std::string line, word;
+    std::getline(h, line);  // title
+    if (line != "Checkpoint file for mflo") return MF_ERR_BAD_ARG;
+    *finest = -1;
+    h >> *finest;
+    std::getline(h, line);
+    if (*finest < 0) return MF_ERR_BAD_ARG;
+    if (*finest > max_finest) return MF_ERR_UNSUPPORTED;
+    isteps.clear();
+    dt.clear();
+    t_new.clear();
+    std::getline(h, line);
+    {
+        std::istringstream ls(line);
+        while (ls >> word) isteps.push_back(std::atoi(word.c_str()));
+    }
+    std::getline(h, line);
+    {
+        std::istringstream ls(line);
+        while (ls >> word) dt.push_back(std::strtod(word.c_str(), nullptr));
+    }
+    std::getline(h, line);
+    {
+        std::istringstream ls(line);
+        while (ls >> word) t_new.push_back(std::strtod(word.c_str(), nullptr));
+    }
+    if ((int)isteps.size() <= *finest || (int)dt.size() <= *finest || (int)t_new.size() <= *finest) return MF_ERR_BAD_ARG;
+    boxes.assign((size_t)*finest + 1, {});
+    for (int l = 0; l <= *finest; ++l)
+        if (!read_boxarray(h, boxes[(size_t)l])) return MF_ERR_BAD_ARG;
+    return MF_OK;
+}
+
+// the state of level l of a checkpoint into a dense array over L's domain
+int read_checkpoint_level(const char *dir, int l, const mf_geom &geom, const std::vector<mf_box> &boxes, int nv,
+                          double *state, const char *const *varnames) {
+    LevelData D{geom, boxes, nv, nullptr};
+    const LevelInfo L = info_of(D);
+    const std::string ld = std::string(dir) + "/Level_" + std::to_string(l);
+    return read_vismf(ld, ld + "/" + var_name(varnames, 0), L, state) ? MF_OK : MF_ERR_BAD_ARG;
+}
+
+}  // namespace mfio
+
+namespace {
+
+mfio::LevelData data_of(const LevelInfo &L, const double *state) { return mfio::LevelData{L.geom, L.boxes, L.nv, state}; }
+
+int write_plotfile(const LevelInfo &L, const double *state, double t, int istep, const char *dir,
+                   const char *const *varnames) {
+    return mfio::write_plotfile_levels({data_of(L, state)}, t, {istep}, 1, dir, varnames);
 }
 
 int write_checkpoint(const LevelInfo &L, const double *state, double t, double dt, int istep, const char *dir,
                      const char *const *varnames) {
-    const std::string d(dir);
-    if (!make_dir(d) || !make_dir(d + "/Level_0")) return MF_ERR_BAD_ARG;
-    const std::string v0 = var_name(varnames, 0);
-    if (!write_vismf(d + "/Level_0/" + v0, v0, L, state)) return MF_ERR_BAD_ARG;
-    std::ofstream h(d + "/Header", std::ios::trunc | std::ios::binary);
-    if (!h) return MF_ERR_BAD_ARG;
-    h << std::setprecision(17);
-    h << "Checkpoint file for mflo\n" << 0 << '\n' << istep << " \n" << dt << " \n" << t << " \n";
-    write_boxarray(h, L.boxes);
-    h << '\n';
-    return h ? MF_OK : MF_ERR_BAD_ARG;
+    return mfio::write_checkpoint_levels({data_of(L, state)}, {t}, {dt}, {istep}, dir, varnames);
 }
 
 int read_checkpoint(const LevelInfo &L, double *state, double *t, double *dt, int *istep, const char *dir,
                     const char *const *varnames) {
-    const std::string d(dir);
-    std::ifstream h(d + "/Header");
-    if (!h) return MF_ERR_BAD_ARG;
-    std::string line;
-    std::getline(h, line);  // title
     int finest = -1;
-    h >> finest;
-    std::getline(h, line);
+    std::vector<int> is;
+    std::vector<double> dts, tn;
+    std::vector<std::vector<mf_box>> bx;
+    int st = mfio::read_checkpoint_header(dir, 0, &finest, is, dts, tn, bx);
+    if (st) return st;
     if (finest != 0) return MF_ERR_UNSUPPORTED;
-    std::getline(h, line);
-    if (std::sscanf(line.c_str(), "%d", istep) != 1) return MF_ERR_BAD_ARG;
-    std::getline(h, line);
-    *dt = std::strtod(line.c_str(), nullptr);
-    std::getline(h, line);
-    *t = std::strtod(line.c_str(), nullptr);
-    std::vector<mf_box> bx;
-    if (!read_boxarray(h, bx)) return MF_ERR_BAD_ARG;
-    if (bx.size() != L.boxes.size() || std::memcmp(bx.data(), L.boxes.data(), bx.size() * sizeof(mf_box)) != 0)
+    *istep = is[0];
+    *dt = dts[0];
+    *t = tn[0];
+    if (bx[0].size() != L.boxes.size() || std::memcmp(bx[0].data(), L.boxes.data(), bx[0].size() * sizeof(mf_box)) != 0)
         return MF_ERR_BAD_ARG;
-    const std::string v0 = var_name(varnames, 0);
-    if (!read_vismf(d + "/Level_0", d + "/Level_0/" + v0, L, state)) return MF_ERR_BAD_ARG;
-    return MF_OK;
+    return mfio::read_checkpoint_level(dir, 0, L.geom, L.boxes, L.nv, state, varnames);
 }
 
 }  // namespace

@@ -424,6 +424,20 @@ struct LdsCell {
 #ifndef MF_FACE_MIN_WAVES
 #define MF_FACE_MIN_WAVES 2
 #endif
+// 1: the per-component reconstruction loop is kept rolled (one component's
+// WENO stencils live at a time: fewer registers, more waves per SIMD)
+#ifndef MF_FACE_CLOOP
+#define MF_FACE_CLOOP 0
+#endif
+// element s of a small register array for a run-time s (a select chain: no
+// dynamic indexing, which would move the array to scratch memory)
+template <int M>
+MFD double pick(const double (&v)[M], int s) {
+    double r = 0.0;
+#pragma unroll
+    for (int q = 0; q < M; ++q) r = (q == s) ? v[q] : r;
+    return r;
+}
 template <class P, int D>
 __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs a, Work w) {
     using N = Num<P>;
@@ -596,7 +610,7 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
 #pragma unroll
                 for (int c = 0; c < NC; ++c)
                     Fp[(int64_t)flux_comp(c, full) * Fs] =
-                        flux_with_terms(c, kZero, false, dm0, dm1, dm2, de0, de1, de2, de3, c >= NCV ? dsp[c - NCV] : 0.0);
+                        flux_with_terms(c, kZero, false, dm0, dm1, dm2, de0, de1, de2, de3, c >= NCV ? pick(dsp, c - NCV) : 0.0);
             } else {
                 bool first = false;
 #pragma unroll
@@ -622,7 +636,11 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
                     double ahalf = std_max(sqrt(gamaL * presL / ul[RHO]), sqrt(gamaR * presR / ur[RHO]));
                     lam = ahalf + veln;
                 }
+#if MF_FACE_CLOOP
+#pragma unroll 1
+#else
 #pragma unroll
+#endif
                 for (int c = 0; c < NC; ++c) {
                     double uc[6], fc[6];
 #pragma unroll
@@ -645,7 +663,7 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
                     double h = kHalf * (fp + lam * up) + kHalf * (fm - lam * um);
                     Fp[(int64_t)flux_comp(c, full) * Fs] =
                         flux_with_terms(c, c < NCV ? h : T.advect_flag[c - NCV] * h, true, dm0, dm1, dm2, de0, de1, de2, de3,
-                                        c >= NCV ? dsp[c - NCV] : 0.0);
+                                        c >= NCV ? pick(dsp, c - NCV) : 0.0);
                 }
             }
             if (full) {

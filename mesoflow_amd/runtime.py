@@ -95,6 +95,9 @@ SIGNATURES = {
     "mf_hier_step": ([C.c_void_p, C.c_int, C.c_int], C.c_int),
     "mf_hier_evolve": ([C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, _P(C.c_int)], C.c_int),
     "mf_hier_info": ([C.c_void_p, _P(C.c_double), _P(C.c_double), _P(C.c_int), C.c_void_p], C.c_int),
+    "mf_hier_write_plotfile": ([C.c_void_p, C.c_char_p, _P(C.c_char_p)], C.c_int),
+    "mf_hier_write_checkpoint": ([C.c_void_p, C.c_char_p, _P(C.c_char_p)], C.c_int),
+    "mf_hier_read_checkpoint": ([C.c_void_p, C.c_char_p, _P(C.c_char_p)], C.c_int),
     "mf_util_copy": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64], C.c_int),
     "mf_util_div_check": ([C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_void_p], C.c_int),
 }
@@ -477,6 +480,18 @@ class Hierarchy:
 
     def boxes(self, lev=1):
         return self.level(lev).boxes()
+
+    def write_plotfile(self, path, varnames=None):
+        check(lib().mf_hier_write_plotfile(self.h, os.fsencode(str(path)), Level._names(varnames)),
+              "mf_hier_write_plotfile")
+
+    def write_checkpoint(self, path, varnames=None):
+        check(lib().mf_hier_write_checkpoint(self.h, os.fsencode(str(path)), Level._names(varnames)),
+              "mf_hier_write_checkpoint")
+
+    def read_checkpoint(self, path, varnames=None):
+        check(lib().mf_hier_read_checkpoint(self.h, os.fsencode(str(path)), Level._names(varnames)),
+              "mf_hier_read_checkpoint")
 
 
 def _hip_memcpy():

@@ -278,3 +278,81 @@ def test_grf_two_level_reactor_full_size():
         fl = a[abi.VFRAC] == 1.0
         assert np.all(a[abi.PRES][fl] > 0)
         assert np.all(a[abi.FLO_NVARS:] >= 0)
+
+
+def _hier_run(cc, fboxes, cs, fs, crit):
+    ctx = runtime.Context(cc)
+    L0 = runtime.Level(ctx)
+    L0.set_state(cs)
+    H = runtime.Hierarchy(L0, 2)
+    H.add_level(fboxes, fs)
+    H.set_regrid(2, *crit, n_error_buf=1, blocking_factor=4, max_grid_size=16, grid_eff=0.7)
+    return ctx, L0, H
+
+
+def test_hierarchy_checkpoint_restart_is_bitwise(tmp_path):
+    """WriteCheckpointFile / ReadCheckpointFile of both levels: a run that
+    writes a checkpoint after 3 coarse steps and takes 3 more equals, bit for
+    bit, a fresh hierarchy that reads the checkpoint and takes the same 3
+    steps (a regrid every 2 steps on both sides)"""
+    from mesoflow_amd import amrexio
+    n = 16
+    cc = all_periodic(n, cases.chop_boxes((n, n, n), 8))
+    s0 = shear_state(cc, amp=0.3)
+    fb0 = [((8, 8, 8), (23, 23, 23))]
+    cs, fs = fine_init(s0, fb0)
+    crit = ([1e30], [0.05], [abi.DENS])
+    chk = str(tmp_path / "chk00003")
+    ctx, L0, H = _hier_run(cc, fb0, cs, fs, crit)
+    H.step(3)
+    H.write_checkpoint(chk)
+    ck = amrexio.read_checkpoint_levels(chk, ctx.nv, [cc.n_cell, tuple(2 * q for q in cc.n_cell)])
+    say("checkpoint: finest %d, istep %s, t %s" % (ck["finest_level"], ck["istep"], ck["time"]))
+    assert ck["finest_level"] == 1 and ck["istep"] == [3, 6]
+    H.step(3)
+    a_c, a_f, a_info, a_boxes = H.get_state(0), H.get_state(1), H.info(), H.boxes(1)
+    _close(ctx, L0, H)
+    ctx2 = runtime.Context(cc)
+    B0 = runtime.Level(ctx2)
+    B0.set_state(cs)
+    HB = runtime.Hierarchy(B0, 2)
+    HB.set_regrid(2, *crit, n_error_buf=1, blocking_factor=4, max_grid_size=16, grid_eff=0.7)
+    HB.read_checkpoint(chk)
+    HB.step(3)
+    b_c, b_f, b_info, b_boxes = HB.get_state(0), HB.get_state(1), HB.info(), HB.boxes(1)
+    m = fine_mask(a_f.shape, a_boxes)
+    say("restart: boxes %d / %d, t %.17g / %.17g" % (len(a_boxes), len(b_boxes), a_info[0], b_info[0]))
+    assert sorted(a_boxes) == sorted(b_boxes)
+    assert np.array_equal(a_c, b_c) and np.array_equal(a_f[:, m], b_f[:, m])
+    assert a_info[0] == b_info[0] and a_info[1] == b_info[1] and a_info[2] == b_info[2]
+    assert np.array_equal(a_info[3], b_info[3])
+    _close(ctx2, B0, HB)
+
+
+def test_hierarchy_plotfile(tmp_path):
+    """WriteMultiLevelPlotfile of the two levels: the Header's levels,
+    refinement ratio and steps, each level's data equal to its state"""
+    from mesoflow_amd import amrexio
+    n = 16
+    cc = all_periodic(n, cases.chop_boxes((n, n, n), 8))
+    s0 = shear_state(cc)
+    fboxes = [((8, 8, 8), (15, 23, 19)), ((16, 8, 8), (21, 15, 19)), ((0, 20, 4), (5, 27, 11))]
+    cs, fs = fine_init(s0, fboxes)
+    ctx = runtime.Context(cc)
+    L0 = runtime.Level(ctx)
+    L0.set_state(cs)
+    H = runtime.Hierarchy(L0, 2)
+    H.add_level(fboxes, fs)
+    H.step(2)
+    p = str(tmp_path / "plt00002")
+    H.write_plotfile(p)
+    pf = amrexio.read_plotfile(p)
+    assert pf["finest_level"] == 1 and pf["ref_ratio"] == [2]
+    assert [lv["level_steps"] for lv in pf["levels"]] == [2, 4]
+    c, f = H.get_state(0), H.get_state(1)
+    assert np.array_equal(pf["levels"][0]["data"], c)
+    m = fine_mask(f.shape, fboxes)
+    got = pf["levels"][1]["data"]
+    assert np.array_equal(got[:, m], f[:, m]) and np.all(np.isnan(got[:, ~m]))
+    assert sorted(pf["levels"][1]["boxes"]) == sorted(fboxes)
+    _close(ctx, L0, H)
