@@ -274,6 +274,7 @@ def amr_two_level_bench(steps):
                            periodic=(0, 1, 1))
     H = runtime.Hierarchy(L0, 2)
     H.regrid(fb)
+    H.autotune()
     nfine = int(sum(np.prod([h[d] - l[d] + 1 for d in range(3)]) for l, h in fb))
     ncrse = int(np.prod(c.n_cell))
     out = {"workload": "two-level porous reactor: 128^3 coarse level in %d uneven boxes (edges 8..64), GRF IB porosity "

@@ -413,6 +413,9 @@ int mf_hier_add_level(void *hier, const mf_box *boxes, int nboxes, const double 
 /* Handle and count of the hierarchy's levels. */
 int mf_hier_level(void *hier, int lev, void **level_out);
 int mf_hier_nlevels(void *hier, int *nlev);
+/* mf_level_autotune for each level of the hierarchy on its own boxes (the
+ * fine level's small boxes take narrower tiles). */
+int mf_hier_autotune(void *hier);
 /* mflo.do_reflux (default 1): Reflux of level 0 after the fine substeps. */
 int mf_hier_set_reflux(void *hier, int do_reflux);
 /* Regrid every regrid_int coarse steps (0: never; Source/mflo_evolve.cpp:

@@ -194,6 +194,7 @@ struct AdjState {
 
 int level_alloc(Ctx *c, const mf_box *boxes, int nboxes, bool partial, Level **out);
 int level_free(Level *L);
+int level_autotune(Level *L);
 
 const KernelSet *pick_kernels(const mf_plugin_params &t) {
     if (const CaseFamily *f = find_case_family(t.kind)) return t.nspec == f->nspec ? f->ks : nullptr;
@@ -1981,13 +1982,28 @@ int mf_level_get_tile(void *level, int cls, int *tx, int *ty, int *tz) {
 int mf_level_autotune(void *level) {
     Level *L = (Level *)level;
     if (!L) return MF_ERR_BAD_ARG;
-    if (L->hier) return MF_ERR_UNSUPPORTED;  // stepped by its hierarchy
+    if (L->hier) return MF_ERR_UNSUPPORTED;  // tuned through its hierarchy (mf_hier_autotune)
+    return level_autotune(L);
+}
+
+}  // extern "C"
+
+namespace {
+// times every kernel class on the level's current state for candidate tile
+// shapes (the last ones narrow, for the small boxes of a refined level) and
+// keeps the fastest; the valid cells of the state are only read
+int level_autotune(Level *L) {
     Ctx *c = L->c;
-    static const int face_cand[3][10][3] = {
-        {{64, 4, 1}, {64, 2, 2}, {32, 16, 1}, {32, 4, 2}, {32, 8, 1}, {64, 1, 4}, {32, 8, 2}, {32, 2, 4}, {32, 4, 4}, {32, 2, 8}},
-        {{32, 8, 1}, {16, 16, 1}, {16, 16, 2}, {32, 4, 2}, {16, 8, 2}, {64, 8, 1}, {32, 16, 1}, {16, 12, 2}, {8, 32, 2}, {16, 32, 1}},
-        {{32, 1, 8}, {16, 2, 8}, {32, 2, 4}, {16, 2, 16}, {16, 1, 16}, {64, 1, 8}, {32, 1, 16}, {16, 2, 12}, {8, 2, 32}, {16, 1, 32}}};
-    static const int cell_cand[8][3] = {{64, 2, 2}, {32, 4, 2}, {64, 4, 1}, {128, 2, 1}, {32, 8, 1}, {16, 4, 4}, {32, 2, 4}, {64, 1, 4}};
+    constexpr int kFaceCand = 13, kCellCand = 11;
+    static const int face_cand[3][kFaceCand][3] = {
+        {{64, 4, 1}, {64, 2, 2}, {32, 16, 1}, {32, 4, 2}, {32, 8, 1}, {64, 1, 4}, {32, 8, 2}, {32, 2, 4}, {32, 4, 4}, {32, 2, 8},
+         {16, 8, 2}, {16, 4, 4}, {8, 8, 4}},
+        {{32, 8, 1}, {16, 16, 1}, {16, 16, 2}, {32, 4, 2}, {16, 8, 2}, {64, 8, 1}, {32, 16, 1}, {16, 12, 2}, {8, 32, 2}, {16, 32, 1},
+         {16, 8, 1}, {8, 16, 2}, {8, 8, 4}},
+        {{32, 1, 8}, {16, 2, 8}, {32, 2, 4}, {16, 2, 16}, {16, 1, 16}, {64, 1, 8}, {32, 1, 16}, {16, 2, 12}, {8, 2, 32}, {16, 1, 32},
+         {16, 1, 8}, {8, 2, 16}, {8, 4, 8}}};
+    static const int cell_cand[kCellCand][3] = {{64, 2, 2}, {32, 4, 2}, {64, 4, 1}, {128, 2, 1}, {32, 8, 1}, {16, 4, 4},
+                                                {32, 2, 4}, {64, 1, 4}, {16, 8, 2}, {8, 8, 4}, {16, 16, 1}};
     HIPCHK(hipStreamSynchronize(c->stream));
     TimeState saved;
     int st = read_ts(L, &saved);
@@ -2000,11 +2016,17 @@ int mf_level_autotune(void *level) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    // ghost cells and cut cells of stage 1 once, so the timed kernels see real data
+    // ghost cells and cut cells of stage 1 once, so the timed kernels see real
+    // data (a level above 0 of a hierarchy: the single-level fill of its
+    // padded array stands in for its two-level fill, whose next stage
+    // overwrites these ghost cells; its cut geometry is made first)
     {
         FillArgs f = fill_args(L, L->arr[0], 1);
         for (int ph = 0; ph <= 3; ++ph) c->ks->fill(f, ph, c->stream);
-        if (L->ncut) c->ks->cut(cut_args(L, L->arr[0]), c->stream);
+        if (L->ncut) {
+            if (L->partial) c->ks->cut_geom(cut_args(L, L->arr[0]), L->cutgeom, L->cutmask, c->stream);
+            c->ks->cut(cut_args(L, L->arr[0]), c->stream);
+        }
     }
     StageArgs a = stage_args(L, 1);
     ResArgs r;
@@ -2028,7 +2050,7 @@ int mf_level_autotune(void *level) {
     const bool lds = use_lds(L);
     for (int cls = 0; cls <= kCellClass && st == MF_OK; ++cls) {
         if (cls < 3 && !lds) continue;
-        const int ncand = cls < 3 ? 10 : 8;
+        const int ncand = cls < 3 ? kFaceCand : kCellCand;
         double best = 1e300;
         int bi[3] = {L->tsh[cls][0], L->tsh[cls][1], L->tsh[cls][2]};
         for (int q = 0; q < ncand; ++q) {
@@ -2061,6 +2083,9 @@ int mf_level_autotune(void *level) {
     HIPCHK(hipMemcpy(c->d_err, &ok, sizeof(int), hipMemcpyHostToDevice));
     return st;
 }
+}  // namespace
+
+extern "C" {
 
 int mf_util_copy(void *ctx, double *dst, const double *src, int64_t n) {
     Ctx *c = (Ctx *)ctx;

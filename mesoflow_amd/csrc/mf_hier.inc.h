@@ -181,16 +181,18 @@ __global__ void k_hier_ct(View ct, View so, View sn, const int *list, int n, int
     }
 }
 
-// cell_cons_interp of the listed fine cells from the coarse patch: the limited
-// slopes of the parent coarse cell, then the child's offsets
-__global__ void k_hier_cf(View f, View ct, const int *list, int n, int nv, int r, double dfac, const TimeState *ts0) {
+// cell_cons_interp of the listed fine cells from the coarse patch: one thread
+// per listed parent coarse cell (list [3][n]) and component forms the limited
+// slopes once and writes the children marked in its mask (bit ir + r (jr + r kr))
+__global__ void k_hier_cf(View f, View ct, const int *list, const uint64_t *mask, int n, int nv, int r, double dfac,
+                          const TimeState *ts0) {
     if (hier_skip(ts0)) return;
     const int64_t tot = (int64_t)n * nv;
     const double rr = (double)r;
     for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
         const int e = (int)(q % n), c = (int)(q / n);
-        const int i = list[e], j = list[n + e], k = list[2 * n + e];
-        const int ic = hfdiv(i, r), jc = hfdiv(j, r), kc = hfdiv(k, r);
+        const int ic = list[e], jc = list[n + e], kc = list[2 * n + e];
+        const uint64_t bits = mask[e];
         const double u = ct(ic, jc, kc, c);
         double sx = hmc_slope(ct(ic - 1, jc, kc, c), u, ct(ic + 1, jc, kc, c));
         double sy = hmc_slope(ct(ic, jc - 1, kc, c), u, ct(ic, jc + 1, kc, c));
@@ -212,10 +214,17 @@ __global__ void k_hier_cf(View f, View ct, const int *list, int n, int nv, int r
         sx *= alpha;
         sy *= alpha;
         sz *= alpha;
-        const double xt = hrn((((double)(i - ic * r) + 0.5) / rr - 0.5) * sx);
-        const double yt = hrn((((double)(j - jc * r) + 0.5) / rr - 0.5) * sy);
-        const double zt = hrn((((double)(k - kc * r) + 0.5) / rr - 0.5) * sz);
-        f(i, j, k, c) = ((u + xt) + yt) + zt;
+        for (int kr = 0; kr < r; ++kr) {
+            const double zt = hrn((((double)kr + 0.5) / rr - 0.5) * sz);
+            for (int jr = 0; jr < r; ++jr) {
+                const double yt = hrn((((double)jr + 0.5) / rr - 0.5) * sy);
+                for (int ir = 0; ir < r; ++ir) {
+                    if (!((bits >> (ir + r * (jr + r * kr))) & 1)) continue;
+                    const double xt = hrn((((double)ir + 0.5) / rr - 0.5) * sx);
+                    f(ic * r + ir, jc * r + jr, kc * r + kr, c) = ((u + xt) + yt) + zt;
+                }
+            }
+        }
     }
 }
 
@@ -353,6 +362,7 @@ struct Hier {
     // fill lists of level 1, [3][n] each
     int *cneed = nullptr, *fcf = nullptr, *fper = nullptr, *fphys[3] = {nullptr, nullptr, nullptr}, *avgc = nullptr;
     int ncneed = 0, nfcf = 0, nfper = 0, nfphys[3] = {0, 0, 0}, navg = 0;
+    uint64_t *fcfm = nullptr;     // child masks of the parents listed in fcf
     double *ct = nullptr;         // coarse patch, padded coarse shape
     RegDev *reg = nullptr;
     double *regv = nullptr;
@@ -380,6 +390,7 @@ void hier_free_lists(Hier *H) {
     int **lists[] = {&H->cneed, &H->fcf, &H->fper, &H->fphys[0], &H->fphys[1], &H->fphys[2], &H->avgc};
     for (int **p : lists)
         if (*p) { hipFree(*p); *p = nullptr; }
+    if (H->fcfm) { hipFree(H->fcfm); H->fcfm = nullptr; }
     if (H->reg) { hipFree(H->reg); H->reg = nullptr; }
     if (H->regv) { hipFree(H->regv); H->regv = nullptr; }
     if (H->tgt) { hipFree(H->tgt); H->tgt = nullptr; }
@@ -387,10 +398,11 @@ void hier_free_lists(Hier *H) {
     for (int p = 0; p < 3; ++p) H->nfphys[p] = 0;
 }
 
-int upload_list(const std::vector<int> &v3, int **dst) {
-    if (v3.empty()) return MF_OK;
-    HIPCHK(hipMalloc(dst, sizeof(int) * v3.size()));
-    HIPCHK(hipMemcpy(*dst, v3.data(), sizeof(int) * v3.size(), hipMemcpyHostToDevice));
+template <class T>
+int upload_list(const std::vector<T> &v, T **dst) {
+    if (v.empty()) return MF_OK;
+    HIPCHK(hipMalloc(dst, sizeof(T) * v.size()));
+    HIPCHK(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
     return MF_OK;
 }
 
@@ -399,6 +411,8 @@ int upload_list(const std::vector<int> &v3, int **dst) {
 // outside the domain by phase, and the coarse cells the interpolation reads
 struct FillSets {
     std::vector<int> cf, per, phys[3], cneed;   // [3][n] after pack()
+    std::vector<int> cfp;                       // parents of the cf cells, [3][n] after pack()
+    std::vector<uint64_t> cfm;                  // their children among the cf cells
 };
 void pack3(std::vector<int> &v) {  // (i,j,k) triples -> [3][n]
     const size_t n = v.size() / 3;
@@ -423,6 +437,7 @@ void build_fill_sets(const int *nc, const int *per, int r, const std::vector<mf_
     const int64_t px = nf[0] + 2 * G, py = nf[1] + 2 * G, pz = nf[2] + 2 * G;
     std::vector<unsigned char> seen((size_t)(px * py * pz), 0);
     std::vector<unsigned char> cseen((size_t)nc[0] * nc[1] * nc[2], 0);
+    std::map<int64_t, size_t> parent;  // parent coarse cell (padded index) -> its entry
     for (const mf_box &R : regions)
         for (int k = R.lo[2]; k <= R.hi[2]; ++k)
             for (int j = R.lo[1]; j <= R.hi[1]; ++j)
@@ -451,6 +466,18 @@ void build_fill_sets(const int *nc, const int *per, int r, const std::vector<mf_
                     S.cf.push_back(i); S.cf.push_back(j); S.cf.push_back(k);
                     int p[3];
                     for (int d = 0; d < 3; ++d) p[d] = (iv[d] >= 0) ? iv[d] / r : -((-iv[d] + r - 1) / r);
+                    {
+                        const int64_t key = (int64_t)(p[0] + G) + (int64_t)(nc[0] + 2 * G) *
+                                            ((int64_t)(p[1] + G) + (int64_t)(nc[1] + 2 * G) * (p[2] + G));
+                        auto it = parent.find(key);
+                        if (it == parent.end()) {
+                            S.cfp.push_back(p[0]); S.cfp.push_back(p[1]); S.cfp.push_back(p[2]);
+                            S.cfm.push_back(0);
+                            it = parent.emplace(key, S.cfm.size() - 1).first;
+                        }
+                        const int bit = (iv[0] - p[0] * r) + r * ((iv[1] - p[1] * r) + r * (iv[2] - p[2] * r));
+                        S.cfm[it->second] |= (uint64_t)1 << bit;
+                    }
                     for (int dk = -1; dk <= 1; ++dk)
                         for (int dj = -1; dj <= 1; ++dj)
                             for (int di = -1; di <= 1; ++di) {
@@ -468,6 +495,7 @@ void build_fill_sets(const int *nc, const int *per, int r, const std::vector<mf_
                             }
                 }
     pack3(S.cf);
+    pack3(S.cfp);
     pack3(S.per);
     pack3(S.cneed);
     for (int p = 0; p < 3; ++p) pack3(S.phys[p]);
@@ -498,12 +526,13 @@ int hier_build(Hier *H) {
     FillSets S;
     build_fill_sets(nc, per, r, Lf->boxes, grown, S);
     int st = upload_list(S.cneed, &H->cneed);
-    if (!st) st = upload_list(S.cf, &H->fcf);
+    if (!st) st = upload_list(S.cfp, &H->fcf);
+    if (!st) st = upload_list(S.cfm, &H->fcfm);
     if (!st) st = upload_list(S.per, &H->fper);
     for (int p = 0; p < 3 && !st; ++p) st = upload_list(S.phys[p], &H->fphys[p]);
     if (st) return st;
     H->ncneed = (int)(S.cneed.size() / 3);
-    H->nfcf = (int)(S.cf.size() / 3);
+    H->nfcf = (int)S.cfm.size();
     H->nfper = (int)(S.per.size() / 3);
     for (int p = 0; p < 3; ++p) H->nfphys[p] = (int)(S.phys[p].size() / 3);
     // coarse cells under the fine level, owners of the coarse cells
@@ -606,7 +635,7 @@ int hier_build(Hier *H) {
 // coarse patch from level 0's old/new data (mode_forced >= 0: that data
 // alone, at time_host), then the interpolation, the images and the boundaries
 void hier_fill_fine(Hier *H, double *dst, double *c_old, double *c_new, int stage, int mode_forced, double time_host,
-                    const int *cneed, int ncneed, const int *fcf, int nfcf, bool ghosts) {
+                    const int *cneed, int ncneed, const int *fcf, const uint64_t *fcfm, int nfcf, bool ghosts) {
     Level *Lc = H->L[0], *Lf = H->L[1];
     Ctx *c = Lc->c, *cf = Lf->c;
     const int nv = c->nv;
@@ -623,8 +652,8 @@ void hier_fill_fine(Hier *H, double *dst, double *c_old, double *c_new, int stag
     for (int ph = 0; ph <= 3; ++ph) c->ks->fill(fa, ph, st);
     const View fv = padded_view(dst, Lf->n, nv);
     if (nfcf)
-        hipLaunchKernelGGL(k_hier_cf, dim3(hgrid((int64_t)nfcf * nv)), dim3(256), 0, st, fv, ctv, fcf, nfcf, nv, H->ratio,
-                           (double)(H->ratio - 1) / (double)(2 * H->ratio), ts0);
+        hipLaunchKernelGGL(k_hier_cf, dim3(hgrid((int64_t)nfcf * nv)), dim3(256), 0, st, fv, ctv, fcf, fcfm, nfcf, nv,
+                           H->ratio, (double)(H->ratio - 1) / (double)(2 * H->ratio), ts0);
     if (!ghosts) return;
     if (H->nfper)
         hipLaunchKernelGGL(k_hier_per, dim3(hgrid((int64_t)H->nfper * nv)), dim3(256), 0, st, fv, H->fper, H->nfper, nv,
@@ -647,7 +676,7 @@ void hier_stage(Hier *H, int lev, int stage, int mode, double *c_old, double *c_
         FillArgs f = fill_args(L, src, stage);
         for (int ph = 0; ph <= 3; ++ph) c->ks->fill(f, ph, st);
     } else {
-        hier_fill_fine(H, src, c_old, c_new, stage, -1, 0.0, H->cneed, H->ncneed, H->fcf, H->nfcf, true);
+        hier_fill_fine(H, src, c_old, c_new, stage, -1, 0.0, H->cneed, H->ncneed, H->fcf, H->fcfm, H->nfcf, true);
     }
     if (L->ncut) {
         CutArgs ca = cut_args(L, src);
@@ -912,8 +941,10 @@ int hier_regrid_to(Hier *H, const mf_box *boxes, int nb, double time, bool in_st
     FillSets S;
     build_fill_sets(Lc->n, Lc->c->geom.is_periodic, H->ratio, old, nbx, S);
     int *cneed = nullptr, *fcf = nullptr;
+    uint64_t *fcfm = nullptr;
     rc = upload_list(S.cneed, &cneed);
-    if (!rc) rc = upload_list(S.cf, &fcf);
+    if (!rc) rc = upload_list(S.cfp, &fcf);
+    if (!rc) rc = upload_list(S.cfm, &fcfm);
     if (!rc && !H->ct) {
         if (hipMalloc(&H->ct, Lc->padded * sizeof(double)) != hipSuccess ||
             hipMemset(H->ct, 0, Lc->padded * sizeof(double)) != hipSuccess)
@@ -928,12 +959,13 @@ int hier_regrid_to(Hier *H, const mf_box *boxes, int nb, double time, bool in_st
             hipLaunchKernelGGL(k_hier_newlevel, dim3(1), dim3(1), 0, st, H->ht, Lf->ts, 1, time, dt0 / H->ratio,
                                had ? 0 : 1);
             hier_fill_fine(H, Lf->arr[0], Lc->arr[0], Lc->arr[0], 1, 1, time, cneed, (int)(S.cneed.size() / 3), fcf,
-                           (int)(S.cf.size() / 3), false);
+                           fcfm, (int)S.cfm.size(), false);
             if (hipStreamSynchronize(st) != hipSuccess) rc = MF_ERR_HIP;
         }
     }
     if (cneed) hipFree(cneed);
     if (fcf) hipFree(fcf);
+    if (fcfm) hipFree(fcfm);
     if (rc) return rc;
     if ((rc = hier_fine_cutlist(Lf))) return rc;
     // the next ComputeDt estimates over the new boxes (inside a step the
@@ -1119,6 +1151,17 @@ int mf_hier_nlevels(void *hier, int *nlev) {
     if (!H || !nlev) return MF_ERR_BAD_ARG;
     *nlev = H->nlev;
     return MF_OK;
+}
+
+int mf_hier_autotune(void *hier) {
+    Hier *H = (Hier *)hier;
+    if (!H) return MF_ERR_BAD_ARG;
+    hier_sync_ctx(H);
+    for (int l = 0; l < H->nlev; ++l) {
+        int rc = level_autotune(H->L[l]);
+        if (rc) return rc;
+    }
+    return hier_build(H);
 }
 
 int mf_hier_set_reflux(void *hier, int do_reflux) {

@@ -89,6 +89,7 @@ SIGNATURES = {
     "mf_hier_level": ([C.c_void_p, C.c_int, _P(C.c_void_p)], C.c_int),
     "mf_hier_nlevels": ([C.c_void_p, _P(C.c_int)], C.c_int),
     "mf_hier_set_reflux": ([C.c_void_p, C.c_int], C.c_int),
+    "mf_hier_autotune": ([C.c_void_p], C.c_int),
     "mf_hier_set_regrid": ([C.c_void_p, C.c_int, _P(C.c_double), _P(C.c_double), _P(C.c_int), C.c_int, C.c_int, C.c_int,
                             C.c_int, C.c_double], C.c_int),
     "mf_hier_regrid": ([C.c_void_p, _P(abi.Box), C.c_int], C.c_int),
@@ -448,6 +449,10 @@ class Hierarchy:
 
     def set_reflux(self, flag):
         check(lib().mf_hier_set_reflux(self.h, int(bool(flag))), "mf_hier_set_reflux")
+
+    def autotune(self):
+        """tile shapes of each level tuned on its own boxes"""
+        check(lib().mf_hier_autotune(self.h), "mf_hier_autotune")
 
     def set_regrid(self, regrid_int, refine_phi, refine_phigrad, comps, n_error_buf=1, blocking_factor=8,
                    max_grid_size=32, grid_eff=0.7):

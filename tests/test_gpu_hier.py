@@ -242,9 +242,10 @@ def test_grf_two_level_reactor_matches_the_oracle():
 def test_grf_two_level_reactor_full_size():
     """the two-level reactor at its full size: 128^3 coarse level in uneven
     boxes, the fine level from the tags (about 10.7M fine cells in about 1300
-    boxes): 6 coarse steps as persistent launches and replayed from a graph
-    give the same state bit for bit, finite, with positive pressure in the
-    fluid and non-negative species"""
+    boxes): 6 coarse steps as persistent launches with the default tiles and
+    replayed from a graph with each level's autotuned tiles give the same
+    state bit for bit, finite, with positive pressure in the fluid and
+    non-negative species"""
     cc = cases.porous_amr()
     s0 = cc.make_state()
     out = []
@@ -255,6 +256,8 @@ def test_grf_two_level_reactor_full_size():
         fb = _tagged_boxes_gpu(L0, **cases.AMR_GRID)
         H = runtime.Hierarchy(L0, 2)
         H.regrid(fb)
+        if mode == abi.MODE_GRAPH:
+            H.autotune()
         ncf = sum(np.prod([h[d] - l[d] + 1 for d in range(3)]) for l, h in fb)
         H.step(1, mode)
         ctx.sync()
