@@ -443,6 +443,33 @@ def porous_reactor(n_cell=(256, 256, 256), nspec=5, nreact=2, geometry="spheres"
     return Case("porous_reactor", t, p, g, make_state, bl, nsteps)
 
 
+def porous_adjoint(n_cell=(128, 128, 128)):
+    """the porous reactor of the adjoint runs: 5 species, 3 reactions with fast
+    kinetics (A = 1e12, 6e11, 3.7e11; Ea 60-64 kJ/mol, k ~ 1e7 1/s at 700 K),
+    so that the rate constants change the outlet flux within a run"""
+    c = porous_reactor(n_cell=n_cell, nspec=5, nreact=3, geometry="spheres", porosity=0.6, rmin=6.0, rmax=14.0,
+                       order_hyp=5, cfl=0.3, seed=20261205)
+    for r in range(3):
+        c.plugin.log_A[r] = np.log(1.0e12) - 0.5 * r
+        c.plugin.Ea[r] = 60.0e3 + 2.0e3 * r
+    return c
+
+
+def perturbed_state(case, seed=1, amp=0.02):
+    """the case's initial state with seeded relative perturbations of the
+    conserved components, the gas species and the velocity (a run whose
+    objective is not a numerical tail)"""
+    s = case.make_state()
+    rng = np.random.default_rng(seed)
+    for q in list(range(5)) + list(range(12, 12 + case.nspec - 1)):
+        s[q] *= 1.0 + amp * rng.uniform(-1.0, 1.0, s[q].shape)
+    s[1:4] += amp * rng.uniform(-1.0, 1.0, s[1:4].shape) * s[0]
+    s[6:9] = s[1:4] / s[0]
+    s[1:4] *= s[11]
+    s[6:9] *= s[11]
+    return s
+
+
 # Two-level porous reactor: a GRF solid at porosity 0.5 on a coarse level of
 # uneven boxes (edges 8..64), the fine level (ratio 2) made from the coarse
 # one where the volume fraction jumps (ErrorEst on vfrac, amr.refine_grad)

@@ -125,26 +125,11 @@ def test_porous512_ns10_all_20_steps():
 # discrete adjoint at 128^3
 # ---------------------------------------------------------------------------
 def _adj_case():
-    c = cases.porous_reactor(n_cell=(128, 128, 128), nspec=5, nreact=3, geometry="spheres", porosity=0.6, rmin=6.0,
-                             rmax=14.0, order_hyp=5, cfl=0.3, seed=20261205)
-    # fast kinetics: A = 1e12, 6e11, 3.7e11; Ea 60-64 kJ/mol (k ~ 1e7 1/s at 700 K), so the rate
-    # constants change the outlet flux within the run
-    for r in range(3):
-        c.plugin.log_A[r] = np.log(1.0e12) - 0.5 * r
-        c.plugin.Ea[r] = 60.0e3 + 2.0e3 * r
-    return c
+    return cases.porous_adjoint()
 
 
 def _perturbed(case, seed=1, amp=0.02):
-    s = case.make_state()
-    rng = np.random.default_rng(seed)
-    for q in list(range(5)) + list(range(12, 12 + case.nspec - 1)):
-        s[q] *= 1.0 + amp * rng.uniform(-1.0, 1.0, s[q].shape)
-    s[1:4] += amp * rng.uniform(-1.0, 1.0, s[1:4].shape) * s[0]
-    s[6:9] = s[1:4] / s[0]
-    s[1:4] *= s[11]
-    s[6:9] *= s[11]
-    return s
+    return cases.perturbed_state(case, seed, amp)
 
 
 def _fixed_dt(case, s0):
