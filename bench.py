@@ -171,7 +171,8 @@ def adjoint_bench(nsteps=200, ncheck_binomial=20):
         bwd = (t2 - t1) / nsteps
         out[name] = {"ncheck": ncheck, "adjoint_forward_ms_per_step": (t1 - t0) / nsteps * 1e3,
                      "backward_ms_per_step": bwd * 1e3, "backward_over_forward": bwd / fwd,
-                     "J": J, "dJ_dlogA": [float(x) for x in gA], "dJ_dinvel": gv}
+                     "J": J, "J_unit": "kg", "dJ_dlogA": [float(x) for x in gA], "dJ_dinvel": gv,
+                     "dlnJ_dlogA": [float(x) / J if J != 0.0 else None for x in gA]}
         log("adjoint %s: %s" % (name, json.dumps(out[name])))
     best = min(out[n]["backward_over_forward"] for n, _ in runs)
     out["adjoint_over_forward"] = best

@@ -15,9 +15,11 @@ namespace mfd {
 
 // ---------------------------------------------------------------------------
 // WENO5-JS (single-division form of Num::weno_plus): gradient w.r.t. s0..s4
-// times gbar, added to sb. weno_minus(s1..s5) == weno_plus(s5,s4,s3,s2,s1).
+// times gbar, added to sb; returns the reconstructed value (Num::weno_plus up
+// to rounding: one reciprocal of the weight sum instead of a division).
+// weno_minus(s1..s5) == weno_plus(s5,s4,s3,s2,s1).
 // ---------------------------------------------------------------------------
-MFD void weno_plus_rev(const double *s, double gbar, double *sb) {
+MFD double weno_plus_rev(const double *s, double gbar, double *sb) {
     const double eps = 1e-6;
     const double t2 = s[0] - 2.0 * s[1] + s[2], q2 = s[0] - 4.0 * s[1] + 3.0 * s[2];
     const double b2 = (13.0 / 12.0) * (t2 * t2) + 0.25 * (q2 * q2);
@@ -50,13 +52,16 @@ MFD void weno_plus_rev(const double *s, double gbar, double *sb) {
     sb[2] += t2b + 3.0 * q2b - 2.0 * t1b + t0b + 3.0 * q0b + 11.0 * v2b + 5.0 * v1b + 2.0 * v0b;
     sb[3] += t1b - r1b - 2.0 * t0b - 4.0 * q0b + 2.0 * v1b + 5.0 * v0b;
     sb[4] += t0b + q0b - v0b;
+    return val;
 }
-// gradient of weno_minus(m[0..4]) (m = s1..s5) times gbar, added to mb
-MFD void weno_minus_rev(const double *m, double gbar, double *mb) {
+// gradient of weno_minus(m[0..4]) (m = s1..s5) times gbar, added to mb;
+// returns the reconstructed value
+MFD double weno_minus_rev(const double *m, double gbar, double *mb) {
     const double r[5] = {m[4], m[3], m[2], m[1], m[0]};
     double rb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    weno_plus_rev(r, gbar, rb);
+    const double val = weno_plus_rev(r, gbar, rb);
     mb[0] += rb[4]; mb[1] += rb[3]; mb[2] += rb[2]; mb[3] += rb[1]; mb[4] += rb[0];
+    return val;
 }
 
 template <class P>
@@ -240,12 +245,13 @@ struct Rev {
                 } else if (c == RHOE) fc[q] = (uc[q] + p) * vn;
                 else fc[q] = uc[q] * vn;
             }
-            // reconstructions of u_c: their adjoints go to u_c directly
-            lamb += kHalf * hb * (N::weno_plus(uc[0], uc[1], uc[2], uc[3], uc[4]) - N::weno_minus(uc[1], uc[2], uc[3], uc[4], uc[5]));
+            // reconstructions of u_c: their adjoints go to u_c directly; the
+            // reverse passes return the reconstructed values lam multiplies
 #pragma unroll
             for (int q = 0; q < 6; ++q) g[q] = 0.0;
-            weno_plus_rev(uc, kHalf * lam * hb, g);
-            weno_minus_rev(uc + 1, -kHalf * lam * hb, g + 1);
+            const double up = weno_plus_rev(uc, kHalf * lam * hb, g);
+            const double um = weno_minus_rev(uc + 1, -kHalf * lam * hb, g + 1);
+            lamb += kHalf * hb * (up - um);
 #pragma unroll
             for (int q = 0; q < 6; ++q) st.add_u(q, c, g[q]);
             // reconstructions of the flux component: through the cell's flux

@@ -122,8 +122,18 @@ static void check_weno() {
         for (int q = 0; q < 5; ++q) { s[q] = urand(-1, 1) * (trial + 1); v[q] = urand(-1, 1); }
         if (trial == 3) for (int q = 0; q < 5; ++q) s[q] = 0.3 + 0.01 * q + 1e-4 * urand(-1, 1);  // smooth data
         double g[5] = {0, 0, 0, 0, 0}, gm[5] = {0, 0, 0, 0, 0};
-        weno_plus_rev(s, w, g);
-        weno_minus_rev(s, w, gm);
+        const double vp = weno_plus_rev(s, w, g);
+        const double vm = weno_minus_rev(s, w, gm);
+        // the reverse passes return the forward reconstructions (to rounding)
+        {
+            const double fp = Num<P>::weno_plus(s[0], s[1], s[2], s[3], s[4]);
+            const double fm = Num<P>::weno_minus(s[0], s[1], s[2], s[3], s[4]);
+            char nm[64];
+            snprintf(nm, sizeof nm, "weno_plus value trial %d", trial);
+            report(nm, vp, fp, 1e-14);
+            snprintf(nm, sizeof nm, "weno_minus value trial %d", trial);
+            report(nm, vm, fm, 1e-14);
+        }
         double ad = 0, adm = 0;
         for (int q = 0; q < 5; ++q) { ad += g[q] * v[q]; adm += gm[q] * v[q]; }
         const double h = 1e-6;
