@@ -468,27 +468,18 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
         const int ox = c0[0] - 3 * e0, oy = c0[1] - 3 * e1, oz = c0[2] - 3 * e2;
         int *lcid = reinterpret_cast<int *>(lds + FS * ncs);
         Acc acc = make_acc(a, b);
+        // staging: one pointer per cell (its cut values or the state), so that
+        // a cut cell loads its values once and the wave does not branch
         for (int q = threadIdx.x; q < ncs; q += blockDim.x) {
             const int ix = q % sx, r = q / sx, iy = r % sy, iz = r / sy;
             const int ci = ox + ix, cj = oy + iy, ck = oz + iz;
             const int cid = acc.cid(ci, cj, ck);
-            const double *sp = acc.s.p + acc.s.off(ci, cj, ck);
-            const int64_t ns = acc.s.ns;
+            const CellPtr cp = cell_ptr(acc, ci, cj, ck, cid);
             double u[NC];
 #pragma unroll
-            for (int n = 0; n < NCV; ++n) u[n] = sp[(int64_t)n * ns];
+            for (int n = 0; n < NCV; ++n) u[n] = cp(n);
 #pragma unroll
-            for (int n = 0; n < NS; ++n) u[NCV + n] = sp[(int64_t)(FLO + n) * ns];
-            double vf = sp[(int64_t)VFRAC * ns];
-            if (cid >= 0) {
-                const double *cp = acc.cutbuf + cid;
-                const int64_t cs = acc.ncut;
-#pragma unroll
-                for (int n = 0; n < NCV; ++n) u[n] = cp[(int64_t)n * cs];
-#pragma unroll
-                for (int n = 0; n < NS; ++n) u[NCV + n] = cp[(int64_t)(FLO + n) * cs];
-                vf = cp[(int64_t)VFRAC * cs];
-            }
+            for (int n = 0; n < NS; ++n) u[NCV + n] = cp(FLO + n);
             double *rec = lds + q * FS;
 #pragma unroll
             for (int c = 0; c < NC; ++c) rec[c] = u[c];
@@ -499,7 +490,7 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
             rec[FVX + 2] = vz;
             rec[FP] = p;
             rec[FG] = P::get_gama_from_rpc(T, u[RHO], p, u + NCV);
-            rec[FVF] = vf;
+            rec[FVF] = cp(VFRAC);
             lcid[q] = cid;
         }
         __syncthreads();
