@@ -507,6 +507,88 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
             const double vfL = recL[FVF], vfR = recR[FVF];
             const double facevfrac = kHalf * (vfL + vfR);
             const bool hyp = facevfrac >= 0.5;
+            // The global loads of the viscous terms (velocities and temperatures
+            // of L and R, cut ids and velocities of the eight transverse
+            // neighbours) are issued first; the hyperbolic fluxes of all
+            // components, computed from LDS, cover their latency; the viscous,
+            // conduction and diffusion terms and the flux stores come last.
+            constexpr int p10 = T1 == 0, p11 = T1 == 1, p12 = T1 == 2;
+            constexpr int p20 = T2 == 0, p21 = T2 == 1, p22 = T2 == 2;
+            int tcid[8];
+            if (hyp) {
+                tcid[0] = acc.cid(iL + p10, jL + p11, kL + p12); tcid[1] = acc.cid(i + p10, j + p11, k + p12);
+                tcid[2] = acc.cid(iL - p10, jL - p11, kL - p12); tcid[3] = acc.cid(i - p10, j - p11, k - p12);
+                tcid[4] = acc.cid(iL + p20, jL + p21, kL + p22); tcid[5] = acc.cid(i + p20, j + p21, k + p22);
+                tcid[6] = acc.cid(iL - p20, jL - p21, kL - p22); tcid[7] = acc.cid(i - p20, j - p21, k - p22);
+            }
+            double pvL[3], pvR[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) { pvL[q] = sel_get(acc, iL, jL, kL, VELX + q, cL); pvR[q] = sel_get(acc, i, j, k, VELX + q, cR); }
+            const double pTL = sel_get(acc, iL, jL, kL, TEMP, cL), pTR = sel_get(acc, i, j, k, TEMP, cR);
+            double hc[NC], tv[8][3];
+            if (hyp) {
+                bool first = false;
+#pragma unroll
+                for (int m = 0; m < 6; ++m) first = first || (lds[(q0 + m * sD) * FS + FVF] < kOne);
+                // stencil records, IB first-order fallback (:233-249)
+                const double *rc0 = lds + (first ? qL : q0) * FS;
+                const double *rc1 = lds + (first ? qL : q0 + sD) * FS;
+                const double *rc4 = lds + (first ? qR : q0 + 4 * sD) * FS;
+                const double *rc5 = lds + (first ? qR : q0 + 5 * sD) * FS;
+                const double *rs[6] = {rc0, rc1, recL, recR, rc4, rc5};
+                // max eigenvalue, flux_schemes.H:10-46 (presR with the left species)
+                double lam;
+                {
+                    double ul[NC], ur[NC];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) { ul[c] = recL[c]; ur[c] = recR[c]; }
+                    double rhoeL = ul[RHOE] - kHalf * (ul[RHOU] * ul[RHOU] + ul[RHOV] * ul[RHOV] + ul[RHOW] * ul[RHOW]) / ul[RHO];
+                    double rhoeR = ur[RHOE] - kHalf * (ur[RHOU] * ur[RHOU] + ur[RHOV] * ur[RHOV] + ur[RHOW] * ur[RHOW]) / ur[RHO];
+                    double presL = P::get_p_from_rec(T, rhoeL, ul[RHO], ul + NCV);
+                    double presR = P::get_p_from_rec(T, rhoeR, ur[RHO], ul + NCV);
+                    double veln = std_max(fabs(recL[FVX + D]), fabs(recR[FVX + D]));
+                    double gamaL = recL[FG], gamaR = recR[FG];
+                    double ahalf = std_max(sqrt(gamaL * presL / ul[RHO]), sqrt(gamaR * presR / ur[RHO]));
+                    lam = ahalf + veln;
+                }
+                {
+                    const int ti[8][3] = {{iL + p10, jL + p11, kL + p12}, {i + p10, j + p11, k + p12},
+                                          {iL - p10, jL - p11, kL - p12}, {i - p10, j - p11, k - p12},
+                                          {iL + p20, jL + p21, kL + p22}, {i + p20, j + p21, k + p22},
+                                          {iL - p20, jL - p21, kL - p22}, {i - p20, j - p21, k - p22}};
+#pragma unroll
+                    for (int m = 0; m < 8; ++m)
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) tv[m][q] = sel_get(acc, ti[m][0], ti[m][1], ti[m][2], VELX + q, tcid[m]);
+                }
+#if MF_FACE_CLOOP
+#pragma unroll 1
+#else
+#pragma unroll
+#endif
+                for (int c = 0; c < NC; ++c) {
+                    double uc[6], fc[6];
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        const double *rq = rs[q];
+                        uc[q] = rq[c];
+                        const double vn = rq[FVX + D];
+                        if (c == RHO) fc[q] = uc[q] * vn;
+                        else if (c >= RHOU && c <= RHOW) {
+                            const double va = rq[FVX + (c - RHOU)];
+                            const double mm = rq[RHO] * va;
+                            fc[q] = (c - RHOU == D) ? (mm * va + rq[FP]) : (mm * vn);
+                        } else if (c == RHOE) fc[q] = (uc[q] + rq[FP]) * vn;
+                        else fc[q] = uc[q] * vn;
+                    }
+                    double fp = N::weno_plus(fc[0], fc[1], fc[2], fc[3], fc[4]);
+                    double fm = N::weno_minus(fc[1], fc[2], fc[3], fc[4], fc[5]);
+                    double up = N::weno_plus(uc[0], uc[1], uc[2], uc[3], uc[4]);
+                    double um = N::weno_minus(uc[1], uc[2], uc[3], uc[4], uc[5]);
+                    const double h = kHalf * (fp + lam * up) + kHalf * (fm - lam * um);
+                    hc[c] = c < NCV ? h : T.advect_flag[c - NCV] * h;
+                }
+            }
             // ---- viscous stress and work (:293-367), conduction (:373-374), species diffusion (:425-428)
             double dm0 = 0.0, dm1 = 0.0, dm2 = 0.0, de0 = 0.0, de1 = 0.0, de2 = 0.0, de3, dsp[NS];
             {
@@ -527,29 +609,18 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
                     thcond = N::harmonic_mean_pos(k0, k1);
                 }
                 if (hyp) {
-                    constexpr int p10 = T1 == 0, p11 = T1 == 1, p12 = T1 == 2;
-                    constexpr int p20 = T2 == 0, p21 = T2 == 1, p22 = T2 == 2;
-                    const int tc0 = acc.cid(iL + p10, jL + p11, kL + p12), tc1 = acc.cid(i + p10, j + p11, k + p12);
-                    const int tc2 = acc.cid(iL - p10, jL - p11, kL - p12), tc3 = acc.cid(i - p10, j - p11, k - p12);
-                    const int tc4 = acc.cid(iL + p20, jL + p21, kL + p22), tc5 = acc.cid(i + p20, j + p21, k + p22);
-                    const int tc6 = acc.cid(iL - p20, jL - p21, kL - p22), tc7 = acc.cid(i - p20, j - p21, k - p22);
                     double g[3][3];  // g[q][m]: d(vel q) along normal (m=0), t1 (m=1), t2 (m=2)
                     double vL[3], vR[3];
 #pragma unroll
                     for (int q = 0; q < 3; ++q) {
-                        const int V = VELX + q;
-                        vL[q] = sel_get(acc, iL, jL, kL, V, cL);
-                        vR[q] = sel_get(acc, i, j, k, V, cR);
+                        vL[q] = pvL[q];
+                        vR[q] = pvR[q];
                         g[q][0] = (vR[q] - vL[q]) * idn;
-                        double top = kHalf * (sel_get(acc, iL + p10, jL + p11, kL + p12, V, tc0) +
-                                              sel_get(acc, i + p10, j + p11, k + p12, V, tc1));
-                        double bot = kHalf * (sel_get(acc, iL - p10, jL - p11, kL - p12, V, tc2) +
-                                              sel_get(acc, i - p10, j - p11, k - p12, V, tc3));
+                        double top = kHalf * (tv[0][q] + tv[1][q]);
+                        double bot = kHalf * (tv[2][q] + tv[3][q]);
                         g[q][1] = (top - bot) * it1;
-                        double frt = kHalf * (sel_get(acc, iL + p20, jL + p21, kL + p22, V, tc4) +
-                                              sel_get(acc, i + p20, j + p21, k + p22, V, tc5));
-                        double bck = kHalf * (sel_get(acc, iL - p20, jL - p21, kL - p22, V, tc6) +
-                                              sel_get(acc, i - p20, j - p21, k - p22, V, tc7));
+                        double frt = kHalf * (tv[4][q] + tv[5][q]);
+                        double bck = kHalf * (tv[6][q] + tv[7][q]);
                         g[q][2] = (frt - bck) * it2;
                     }
                     // axis -> gradient slot (GET_XDIR/GET_YDIR/GET_ZDIR, globalDefines.H:49-51)
@@ -581,7 +652,7 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
                     de1 = t1 * (kHalf * (vL[1] + vR[1]));
                     de2 = t2 * (kHalf * (vL[2] + vR[2]));
                 }
-                const double dTdn = (sel_get(acc, i, j, k, TEMP, cR) - sel_get(acc, iL, jL, kL, TEMP, cL)) * idn;
+                const double dTdn = (pTR - pTL) * idn;
                 de3 = thcond * dTdn;
                 double dl[NS], dr[NS];
                 LdsCell<NC> lc{recL, &acc, iL, jL, kL, cL};
@@ -597,66 +668,10 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
             const View &F = b.flux[D];
             double *Fp = F.p + F.off(i, j, k);
             const int64_t Fs = F.ns;
-            if (!hyp) {
 #pragma unroll
-                for (int c = 0; c < NC; ++c)
-                    Fp[(int64_t)flux_comp(c, full) * Fs] =
-                        flux_with_terms(c, kZero, false, dm0, dm1, dm2, de0, de1, de2, de3, c >= NCV ? pick(dsp, c - NCV) : 0.0);
-            } else {
-                bool first = false;
-#pragma unroll
-                for (int m = 0; m < 6; ++m) first = first || (lds[(q0 + m * sD) * FS + FVF] < kOne);
-                // stencil records, IB first-order fallback (:233-249)
-                const double *rc0 = lds + (first ? qL : q0) * FS;
-                const double *rc1 = lds + (first ? qL : q0 + sD) * FS;
-                const double *rc4 = lds + (first ? qR : q0 + 4 * sD) * FS;
-                const double *rc5 = lds + (first ? qR : q0 + 5 * sD) * FS;
-                const double *rs[6] = {rc0, rc1, recL, recR, rc4, rc5};
-                // max eigenvalue, flux_schemes.H:10-46 (presR with the left species)
-                double lam;
-                {
-                    double ul[NC], ur[NC];
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) { ul[c] = recL[c]; ur[c] = recR[c]; }
-                    double rhoeL = ul[RHOE] - kHalf * (ul[RHOU] * ul[RHOU] + ul[RHOV] * ul[RHOV] + ul[RHOW] * ul[RHOW]) / ul[RHO];
-                    double rhoeR = ur[RHOE] - kHalf * (ur[RHOU] * ur[RHOU] + ur[RHOV] * ur[RHOV] + ur[RHOW] * ur[RHOW]) / ur[RHO];
-                    double presL = P::get_p_from_rec(T, rhoeL, ul[RHO], ul + NCV);
-                    double presR = P::get_p_from_rec(T, rhoeR, ur[RHO], ul + NCV);
-                    double veln = std_max(fabs(recL[FVX + D]), fabs(recR[FVX + D]));
-                    double gamaL = recL[FG], gamaR = recR[FG];
-                    double ahalf = std_max(sqrt(gamaL * presL / ul[RHO]), sqrt(gamaR * presR / ur[RHO]));
-                    lam = ahalf + veln;
-                }
-#if MF_FACE_CLOOP
-#pragma unroll 1
-#else
-#pragma unroll
-#endif
-                for (int c = 0; c < NC; ++c) {
-                    double uc[6], fc[6];
-#pragma unroll
-                    for (int q = 0; q < 6; ++q) {
-                        const double *rq = rs[q];
-                        uc[q] = rq[c];
-                        const double vn = rq[FVX + D];
-                        if (c == RHO) fc[q] = uc[q] * vn;
-                        else if (c >= RHOU && c <= RHOW) {
-                            const double va = rq[FVX + (c - RHOU)];
-                            const double mm = rq[RHO] * va;
-                            fc[q] = (c - RHOU == D) ? (mm * va + rq[FP]) : (mm * vn);
-                        } else if (c == RHOE) fc[q] = (uc[q] + rq[FP]) * vn;
-                        else fc[q] = uc[q] * vn;
-                    }
-                    double fp = N::weno_plus(fc[0], fc[1], fc[2], fc[3], fc[4]);
-                    double fm = N::weno_minus(fc[1], fc[2], fc[3], fc[4], fc[5]);
-                    double up = N::weno_plus(uc[0], uc[1], uc[2], uc[3], uc[4]);
-                    double um = N::weno_minus(uc[1], uc[2], uc[3], uc[4], uc[5]);
-                    double h = kHalf * (fp + lam * up) + kHalf * (fm - lam * um);
-                    Fp[(int64_t)flux_comp(c, full) * Fs] =
-                        flux_with_terms(c, c < NCV ? h : T.advect_flag[c - NCV] * h, true, dm0, dm1, dm2, de0, de1, de2, de3,
-                                        c >= NCV ? pick(dsp, c - NCV) : 0.0);
-                }
-            }
+            for (int c = 0; c < NC; ++c)
+                Fp[(int64_t)flux_comp(c, full) * Fs] =
+                    flux_with_terms(c, hyp ? hc[c] : kZero, hyp, dm0, dm1, dm2, de0, de1, de2, de3, c >= NCV ? pick(dsp, c - NCV) : 0.0);
             if (full) {
 #pragma unroll
                 for (int n = NCV; n < FLO; ++n) F(i, j, k, n) = kZero;

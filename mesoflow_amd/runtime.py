@@ -110,6 +110,16 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libmesoflow_amd.so is not built; run __graft_entry__.build()")
+        # torch ships its own HIP runtime under the same sonames
+        # (libamdhip64.so.7, libhsa-runtime64.so.1) as the ROCm one this
+        # library links: whichever loads first serves the whole process. Loading
+        # torch first keeps torch on the runtime it was built with (this library
+        # runs on it as well); loading this library first would bind torch to the
+        # system runtime, on which torch then finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (args, res) in SIGNATURES.items():
             f = getattr(L, name)
@@ -272,11 +282,11 @@ class Level:
         self.ctx.sync()
         n = [v.end[d] - v.begin[d] for d in range(3)]
         count = v.nstride * v.ncomp
-        buf = np.empty(count)
-        import ctypes
-        hip = _hip_memcpy()
-        hip(buf.ctypes.data_as(C.c_void_p), C.c_void_p(v.p), C.c_size_t(count * 8), 2)
-        return buf.reshape(v.ncomp, n[2], n[1], n[0])
+        # device copy into a torch buffer on the level's stream, then to the host
+        t = torch.empty(count, dtype=torch.float64, device="cuda")
+        check(lib().mf_util_copy(self.ctx.h, C.c_void_p(t.data_ptr()), C.c_void_p(v.p), C.c_int64(count)), "mf_util_copy")
+        self.ctx.sync()
+        return t.cpu().numpy().reshape(v.ncomp, n[2], n[1], n[0])
 
     def info(self):
         t, dt, ist = C.c_double(), C.c_double(), C.c_int()
@@ -364,7 +374,6 @@ class Level:
         return out
 
 
-_HIPMEMCPY = None
 
 
 def _box_array(boxes):
@@ -499,12 +508,3 @@ class Hierarchy:
               "mf_hier_read_checkpoint")
 
 
-def _hip_memcpy():
-    global _HIPMEMCPY
-    if _HIPMEMCPY is None:
-        h = C.CDLL("libamdhip64.so")
-        f = h.hipMemcpy
-        f.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-        f.restype = C.c_int
-        _HIPMEMCPY = f
-    return _HIPMEMCPY
