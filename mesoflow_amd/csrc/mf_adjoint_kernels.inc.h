@@ -149,6 +149,22 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
             if (!any) continue;
             const double facevfrac = kHalf * (vfL + vfR);
             const bool hyp = facevfrac >= 0.5;
+            // pass 2's global loads first (cut ids, L/R velocities, transverse
+            // cut ids); the LLF reverse covers their latency
+            constexpr int p10 = T1 == 0, p11 = T1 == 1, p12 = T1 == 2;
+            constexpr int p20 = T2 == 0, p21 = T2 == 1, p22 = T2 == 2;
+            const int cL = acc.cid(iL, jL, kL), cR = acc.cid(i, j, k);
+            const int ti[8][3] = {{iL + p10, jL + p11, kL + p12}, {i + p10, j + p11, k + p12},
+                                  {iL - p10, jL - p11, kL - p12}, {i - p10, j - p11, k - p12},
+                                  {iL + p20, jL + p21, kL + p22}, {i + p20, j + p21, k + p22},
+                                  {iL - p20, jL - p21, kL - p22}, {i - p20, j - p21, k - p22}};
+            int tc[8];
+            if (hyp) {
+#pragma unroll
+                for (int m = 0; m < 8; ++m) tc[m] = acc.cid(ti[m][0], ti[m][1], ti[m][2]);
+            }
+            const CellPtr pL = cell_ptr(acc, iL, jL, kL, cL), pR = cell_ptr(acc, i, j, k, cR);
+            const double vL[3] = {pL(VELX), pL(VELY), pL(VELZ)}, vR[3] = {pR(VELX), pR(VELY), pR(VELZ)};
             if (hyp) {
                 bool first = false;
 #pragma unroll
@@ -167,8 +183,6 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
                 R::template face_hyp_rev<D>(T, st);
             }
             // pass 2: viscous stress and work, conduction, species diffusion
-            const int cL = acc.cid(iL, jL, kL), cR = acc.cid(i, j, k);
-            const CellPtr pL = cell_ptr(acc, iL, jL, kL, cL), pR = cell_ptr(acc, i, j, k, cR);
             double visc, thcond;
             {
                 double v0, k0, v1, k1;
@@ -189,15 +203,6 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
 #pragma unroll
                 for (int s = 0; s < NS; ++s) Ds[s] = N::harmonic_mean(dl[s], dr[s]);
             }
-            const double vL[3] = {pL(VELX), pL(VELY), pL(VELZ)}, vR[3] = {pR(VELX), pR(VELY), pR(VELZ)};
-            constexpr int p10 = T1 == 0, p11 = T1 == 1, p12 = T1 == 2;
-            constexpr int p20 = T2 == 0, p21 = T2 == 1, p22 = T2 == 2;
-            // transverse neighbours: L+t1, R+t1, L-t1, R-t1, L+t2, R+t2, L-t2, R-t2
-            const int ti[8][3] = {{iL + p10, jL + p11, kL + p12}, {i + p10, j + p11, k + p12},
-                                  {iL - p10, jL - p11, kL - p12}, {i - p10, j - p11, k - p12},
-                                  {iL + p20, jL + p21, kL + p22}, {i + p20, j + p21, k + p22},
-                                  {iL - p20, jL - p21, kL - p22}, {i - p20, j - p21, k - p22}};
-            int tc[8];
             double G[3][3];
 #pragma unroll
             for (int q = 0; q < 3; ++q) G[q][0] = G[q][1] = G[q][2] = 0.0;
@@ -205,7 +210,6 @@ __global__ __launch_bounds__(128) void k_faces_adj(AdjFaceArgs aa, Work w) {
                 double tv[8][3];
 #pragma unroll
                 for (int m = 0; m < 8; ++m) {
-                    tc[m] = acc.cid(ti[m][0], ti[m][1], ti[m][2]);
 #pragma unroll
                     for (int q = 0; q < 3; ++q) tv[m][q] = acc.get(ti[m][0], ti[m][1], ti[m][2], VELX + q, tc[m]);
                 }
