@@ -509,9 +509,10 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
             const bool hyp = facevfrac >= 0.5;
             // The global loads of the viscous terms (velocities and temperatures
             // of L and R, cut ids and velocities of the eight transverse
-            // neighbours) are issued first; the hyperbolic fluxes of all
-            // components, computed from LDS, cover their latency; the viscous,
-            // conduction and diffusion terms and the flux stores come last.
+            // neighbours) are issued first; the hyperbolic fluxes of the flow
+            // components, computed from LDS, cover their latency; then the
+            // viscous, conduction and diffusion terms, the flow fluxes' stores,
+            // and the species components.
             constexpr int p10 = T1 == 0, p11 = T1 == 1, p12 = T1 == 2;
             constexpr int p20 = T2 == 0, p21 = T2 == 1, p22 = T2 == 2;
             int tcid[8];
@@ -525,7 +526,8 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
 #pragma unroll
             for (int q = 0; q < 3; ++q) { pvL[q] = sel_get(acc, iL, jL, kL, VELX + q, cL); pvR[q] = sel_get(acc, i, j, k, VELX + q, cR); }
             const double pTL = sel_get(acc, iL, jL, kL, TEMP, cL), pTR = sel_get(acc, i, j, k, TEMP, cR);
-            double hc[NC], tv[8][3];
+            double hc[NC], tv[8][3], lam = 0.0;
+            const double *rs[6] = {recL, recL, recL, recR, recR, recR};
             if (hyp) {
                 bool first = false;
 #pragma unroll
@@ -535,9 +537,8 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
                 const double *rc1 = lds + (first ? qL : q0 + sD) * FS;
                 const double *rc4 = lds + (first ? qR : q0 + 4 * sD) * FS;
                 const double *rc5 = lds + (first ? qR : q0 + 5 * sD) * FS;
-                const double *rs[6] = {rc0, rc1, recL, recR, rc4, rc5};
+                rs[0] = rc0; rs[1] = rc1; rs[4] = rc4; rs[5] = rc5;
                 // max eigenvalue, flux_schemes.H:10-46 (presR with the left species)
-                double lam;
                 {
                     double ul[NC], ur[NC];
 #pragma unroll
@@ -561,12 +562,17 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
 #pragma unroll
                         for (int q = 0; q < 3; ++q) tv[m][q] = sel_get(acc, ti[m][0], ti[m][1], ti[m][2], VELX + q, tcid[m]);
                 }
+            }
+            // flow components (the reconstruction loop is written out twice:
+            // the species copy below runs after the viscous terms have freed
+            // the transverse velocities' registers)
+            if (hyp) {
 #if MF_FACE_CLOOP
 #pragma unroll 1
 #else
 #pragma unroll
 #endif
-                for (int c = 0; c < NC; ++c) {
+                for (int c = 0; c < NCV; ++c) {
                     double uc[6], fc[6];
 #pragma unroll
                     for (int q = 0; q < 6; ++q) {
@@ -669,9 +675,42 @@ __global__ __launch_bounds__(512, MF_FACE_MIN_WAVES) void k_faces_lds(StageArgs 
             double *Fp = F.p + F.off(i, j, k);
             const int64_t Fs = F.ns;
 #pragma unroll
-            for (int c = 0; c < NC; ++c)
+            for (int c = 0; c < NCV; ++c)
+                Fp[(int64_t)flux_comp(c, full) * Fs] = flux_with_terms(c, hyp ? hc[c] : kZero, hyp, dm0, dm1, dm2, de0, de1, de2, de3, 0.0);
+            // species components
+            if (hyp) {
+#if MF_FACE_CLOOP
+#pragma unroll 1
+#else
+#pragma unroll
+#endif
+                for (int c = NCV; c < NC; ++c) {
+                    double uc[6], fc[6];
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) {
+                        const double *rq = rs[q];
+                        uc[q] = rq[c];
+                        const double vn = rq[FVX + D];
+                        if (c == RHO) fc[q] = uc[q] * vn;
+                        else if (c >= RHOU && c <= RHOW) {
+                            const double va = rq[FVX + (c - RHOU)];
+                            const double mm = rq[RHO] * va;
+                            fc[q] = (c - RHOU == D) ? (mm * va + rq[FP]) : (mm * vn);
+                        } else if (c == RHOE) fc[q] = (uc[q] + rq[FP]) * vn;
+                        else fc[q] = uc[q] * vn;
+                    }
+                    double fp = N::weno_plus(fc[0], fc[1], fc[2], fc[3], fc[4]);
+                    double fm = N::weno_minus(fc[1], fc[2], fc[3], fc[4], fc[5]);
+                    double up = N::weno_plus(uc[0], uc[1], uc[2], uc[3], uc[4]);
+                    double um = N::weno_minus(uc[1], uc[2], uc[3], uc[4], uc[5]);
+                    const double h = kHalf * (fp + lam * up) + kHalf * (fm - lam * um);
+                    hc[c] = c < NCV ? h : T.advect_flag[c - NCV] * h;
+                }
+            }
+#pragma unroll
+            for (int c = NCV; c < NC; ++c)
                 Fp[(int64_t)flux_comp(c, full) * Fs] =
-                    flux_with_terms(c, hyp ? hc[c] : kZero, hyp, dm0, dm1, dm2, de0, de1, de2, de3, c >= NCV ? pick(dsp, c - NCV) : 0.0);
+                    flux_with_terms(c, hyp ? hc[c] : kZero, hyp, dm0, dm1, dm2, de0, de1, de2, de3, pick(dsp, c - NCV));
             if (full) {
 #pragma unroll
                 for (int n = NCV; n < FLO; ++n) F(i, j, k, n) = kZero;
