@@ -116,9 +116,10 @@ def _two_level_step(reflux, out=None):
         ctx_c.sync()
         _ts()
         # the level's padded array (ghost-filled) as this stage's Sborder
-        hip = runtime._hip_memcpy()
         assert pv.nstride * pv.ncomp == sb_c.numel()
-        assert hip(C.c_void_p(sb_c.data_ptr()), C.c_void_p(pv.p), C.c_size_t(sb_c.numel() * 8), 3) == 0
+        runtime.check(runtime.lib().mf_util_copy(ctx_c.h, C.c_void_p(sb_c.data_ptr()), C.c_void_p(pv.p),
+                                                 C.c_int64(sb_c.numel())), "mf_util_copy")
+        ctx_c.sync()
         ds, fl = _rhs(ctx_c, *dom, sb_c, (-3, -3, -3), t0 if stage == 1 else t0 + dt)
         src, src_lo = (sb_c, (-3, -3, -3)) if stage == 1 else (uc_old, (0, 0, 0))
         _combine(ctx_c, stage, *dom, s_new, src, src_lo, ds, dt)
