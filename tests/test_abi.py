@@ -64,3 +64,17 @@ def test_no_device_here_is_reported():
     with pytest.raises(abi.MFError) as e:
         runtime.Context(c)
     assert e.value.status == abi.MF_ERR_NO_DEVICE
+
+
+def test_library_loads_after_torch():
+    """runtime.lib() imports torch before it loads the library, so a process
+    that uses both keeps torch's HIP runtime (the two share sonames)."""
+    import sys
+    code = ("import sys; from mesoflow_amd import runtime; assert 'torch' not in sys.modules; "
+            "runtime.lib(); assert 'torch' in sys.modules; "
+            "maps = open('/proc/self/maps').read(); "
+            "import torch, os; tlib = os.path.realpath(os.path.join(os.path.dirname(torch.__file__), 'lib', 'libamdhip64.so')); "
+            "hips = {l.split()[-1] for l in maps.splitlines() if 'libamdhip64' in l}; "
+            "print(sorted(hips)); assert hips == {tlib}, hips")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
