@@ -205,6 +205,8 @@ struct KernelSet {
     void (*chem)(const ResArgs &, const Work &, int grid, int block, hipStream_t);
     void (*cut)(const CutArgs &, hipStream_t);
     void (*cut_geom)(const CutArgs &, double *geom, unsigned *mask, hipStream_t);
+    // the same for the listed cut cells only (indices into the cut list)
+    void (*cut_geom_sub)(const CutArgs &, double *geom, unsigned *mask, const int *sub, int nsub, hipStream_t);
     void (*cut_box)(const CutArgs &, hipStream_t);
     void (*faces)(const StageArgs &, const Work &, int dir, int grid, int block, hipStream_t);
     // LDS-staged WENO5 sweeps (order_hyp = 5); lds_bytes(tile) gives the shared memory of a tile

@@ -143,6 +143,11 @@ struct Level {
     double *cutgeom = nullptr;      // static geometry of the cut cells ([4][ncut])
     unsigned *cutmask = nullptr;    // fluid-neighbour masks ([4][ncut])
     int ncut = 0;
+    // level above 0 of a hierarchy: the cut cells (list indices) whose 5^3
+    // neighbourhood reaches a cell its boxes do not cover; only their
+    // geometry changes with the two-level fill
+    int *geomsub = nullptr;
+    int ngeomsub = 0;
     int geom_gen = 0;  // incremented whenever the cut-cell list and geometry are rebuilt
     double *partials = nullptr;
     size_t partials_cap = 0;
@@ -535,11 +540,29 @@ int build_cutlist(Level *L, const double *vfrac_host) {
             lk.push_back((int)(q / ((int64_t)nx * ny)));
         }
     }
+    // cut cells whose neighbourhood (radius 2) leaves the covered region or the domain
+    std::vector<int> sub;
+    if (L->partial) {
+        for (int q = 0; q < nc; ++q) {
+            bool edge = false;
+            for (int n = -2; n <= 2 && !edge; ++n)
+                for (int m = -2; m <= 2 && !edge; ++m)
+                    for (int l = -2; l <= 2 && !edge; ++l) {
+                        const int i = li[q] + l, j = lj[q] + m, k = lk[q] + n;
+                        if (i < 0 || i >= nx || j < 0 || j >= ny || k < 0 || k >= nz ||
+                            !cov[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)])
+                            edge = true;
+                    }
+            if (edge) sub.push_back(q);
+        }
+    }
     HIPCHK(hipStreamSynchronize(L->c->stream));
     if (L->cutlist) { hipFree(L->cutlist); L->cutlist = nullptr; }
     if (L->cutbuf) { hipFree(L->cutbuf); L->cutbuf = nullptr; }
     if (L->cutgeom) { hipFree(L->cutgeom); L->cutgeom = nullptr; }
     if (L->cutmask) { hipFree(L->cutmask); L->cutmask = nullptr; }
+    if (L->geomsub) { hipFree(L->geomsub); L->geomsub = nullptr; }
+    L->ngeomsub = 0;
     L->ncut = nc;
     ++L->geom_gen;
     HIPCHK(hipMemcpy(L->cutid, id.data(), sizeof(int) * ncell, hipMemcpyHostToDevice));
@@ -555,14 +578,23 @@ int build_cutlist(Level *L, const double *vfrac_host) {
         HIPCHK(hipMalloc(&L->cutgeom, sizeof(double) * 6 * (size_t)nc));
         HIPCHK(hipMalloc(&L->cutmask, sizeof(unsigned) * 4 * (size_t)nc));
         // the geometry reads vfrac of the 5^3 neighbourhoods: fill the ghost
-        // layers first (a partly covered level recomputes it after each of its
-        // two-level fills, whose ghost vfrac comes from the coarse level)
+        // layers first. A partly covered level computes it here from its valid
+        // data for the cut cells whose neighbourhoods its boxes cover, and
+        // after each of its two-level fills for the listed others (their ghost
+        // vfrac comes from the coarse level)
         if (!L->partial) {
             FillArgs f = fill_args(L, L->arr[0], 1);
             f.ts = nullptr;
             f.time_host = 0.0;
             for (int ph = 0; ph <= 3; ++ph) L->c->ks->fill(f, ph, L->c->stream);
             L->c->ks->cut_geom(cut_args(L, L->arr[0]), L->cutgeom, L->cutmask, L->c->stream);
+        } else {
+            L->c->ks->cut_geom(cut_args(L, L->arr[0]), L->cutgeom, L->cutmask, L->c->stream);
+            if (!sub.empty()) {
+                HIPCHK(hipMalloc(&L->geomsub, sizeof(int) * sub.size()));
+                HIPCHK(hipMemcpy(L->geomsub, sub.data(), sizeof(int) * sub.size(), hipMemcpyHostToDevice));
+                L->ngeomsub = (int)sub.size();
+            }
         }
         HIPCHK(hipGetLastError());
     }
@@ -1593,6 +1625,7 @@ int level_free(Level *L) {
     if (L->cutbuf) hipFree(L->cutbuf);
     if (L->cutgeom) hipFree(L->cutgeom);
     if (L->cutmask) hipFree(L->cutmask);
+    if (L->geomsub) hipFree(L->geomsub);
     if (L->partials) hipFree(L->partials);
     if (L->d_box) hipFree(L->d_box);
     if (L->ts) hipFree(L->ts);

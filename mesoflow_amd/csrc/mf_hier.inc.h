@@ -680,7 +680,7 @@ void hier_stage(Hier *H, int lev, int stage, int mode, double *c_old, double *c_
     }
     if (L->ncut) {
         CutArgs ca = cut_args(L, src);
-        if (L->partial) c->ks->cut_geom(ca, L->cutgeom, L->cutmask, st);
+        if (L->partial) c->ks->cut_geom_sub(ca, L->cutgeom, L->cutmask, L->geomsub, L->ngeomsub, st);
         c->ks->cut(ca, st);
     }
     StageArgs a = stage_args(L, stage);

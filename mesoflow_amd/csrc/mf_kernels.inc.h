@@ -1490,6 +1490,21 @@ __global__ __launch_bounds__(128) void k_cut_geom(CutArgs a, double *geom, unsig
     for (int w = 0; w < 4; ++w) mask[(int64_t)w * a.ncut + q] = mb[w];
 }
 
+// the static geometry of the cut cells sub[0..nsub) of the list
+template <class P>
+__global__ __launch_bounds__(128) void k_cut_geom_sub(CutArgs a, double *geom, unsigned *mask, const int *sub, int nsub) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsub) return;
+    const int q = sub[t];
+    double g[6];
+    unsigned mb[4];
+    cut_geom_cell(a, a.list[q], a.list[a.ncut + q], a.list[2 * a.ncut + q], g, mb);
+#pragma unroll
+    for (int d = 0; d < 6; ++d) geom[(int64_t)d * a.ncut + q] = g[d];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) mask[(int64_t)w * a.ncut + q] = mb[w];
+}
+
 // update_cutcells of the level's cut-cell list (ib_utils.H:271-436), one thread
 // per cut cell, with the static geometry of k_cut_geom: the IDW sums run over
 // the fluid neighbours of the mask in the reference's loop order.
@@ -1752,6 +1767,10 @@ struct Launch {
         if (a.ncut <= 0) return;
         hipLaunchKernelGGL(k_cut_geom<P>, dim3((a.ncut + 127) / 128), dim3(128), 0, st, a, geom, mask);
     }
+    static void cut_geom_sub(const CutArgs &a, double *geom, unsigned *mask, const int *sub, int nsub, hipStream_t st) {
+        if (nsub <= 0) return;
+        hipLaunchKernelGGL(k_cut_geom_sub<P>, dim3((nsub + 127) / 128), dim3(128), 0, st, a, geom, mask, sub, nsub);
+    }
     static void cut_box(const CutArgs &a, hipStream_t st) {
         int64_t n = (int64_t)(a.hi[0] - a.lo[0] + 1) * (a.hi[1] - a.lo[1] + 1) * (a.hi[2] - a.lo[2] + 1);
         if (n <= 0) return;
@@ -1820,6 +1839,7 @@ struct Launch {
         k.chem = &chem;
         k.cut = &cut;
         k.cut_geom = &cut_geom;
+        k.cut_geom_sub = &cut_geom_sub;
         k.cut_box = &cut_box;
         k.faces = &faces;
         k.faces_lds = &faces_lds;
