@@ -277,12 +277,25 @@ def amr_two_level_bench(steps):
     H = runtime.Hierarchy(L0, 2)
     H.regrid(fb)
     H.autotune()
+    import ctypes as C
+
+    def tiles_of(h):
+        out_t = []
+        for q in range(4):
+            a, b, cc = C.c_int(), C.c_int(), C.c_int()
+            runtime.check(runtime.lib().mf_level_get_tile(h, q, C.byref(a), C.byref(b), C.byref(cc)), "mf_level_get_tile")
+            out_t.append("%d,%d,%d" % (a.value, b.value, cc.value))
+        return ";".join(out_t)
+    h1 = C.c_void_p()
+    runtime.check(runtime.lib().mf_hier_level(H.h, 1, C.byref(h1)), "mf_hier_level")
+    tiles = {"coarse": tiles_of(L0.h), "fine": tiles_of(h1)}
     nfine = int(sum(np.prod([h[d] - l[d] + 1 for d in range(3)]) for l, h in fb))
     ncrse = int(np.prod(c.n_cell))
     out = {"workload": "two-level porous reactor: 128^3 coarse level in %d uneven boxes (edges 8..64), GRF IB porosity "
                        "0.5, 5 species, 2 reactions, WENO5; fine level ratio 2 subcycled, %d boxes, %d cells "
                        "(%.1f%% of 256^3), from the vfrac tags" % (len(c.boxes), len(fb), nfine, 100.0 * nfine / 256 ** 3),
-           "coarse_boxes": len(c.boxes), "fine_boxes": len(fb), "fine_cells": nfine, "steps": steps}
+           "coarse_boxes": len(c.boxes), "fine_boxes": len(fb), "fine_cells": nfine, "steps": steps,
+           "tiles": tiles}
     for name, m in (("batched", abi.MODE_BATCHED), ("graph", abi.MODE_GRAPH)):
         H.step(1, m)
         torch.cuda.synchronize()
